@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark: gates/s and % of HBM roofline of the plain FP64 apply on the
+30-qubit brickwork random circuit (16 GiB state), one MI355X.
+
+A step is one brickwork layer: u3 on every qubit (3 SingleOps each), rzz on
+alternating nearest-neighbour pairs, one controlled x -- 105 or 106
+reference SingleOps.  --steps K layers are timed after --warmup W layers
+(default 40: the depth-40 circuit).  `value` counts reference SingleOps
+(u3 = 3), the unit the plain form launches one kernel for.
+
+The roofline object reports the kernel class that dominates device time,
+from HIP events recorded on the register's stream around every launch in
+the timed region: achieved = algorithmic bytes 2 * 16 B * 2^(n - #controls)
+per launch / average launch time.  The CPU baseline times the oracle (the C
+restatement of qvnt's out-of-place gather with its OpenMP twin of the Rayon
+loop) on the host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "gates/s + % HBM roofline at 30q FP64; served circuits/s at p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, budget_s):
+    """Oracle (qvnt's algorithm in C, OpenMP) on the first SingleOps of
+    brickwork layer 0 at full size, grown until ~budget_s of work."""
+    import oracle
+    from qvnt_amd import circuits
+    threads = min(os.cpu_count() or 1, 16)
+    specs = circuits.brickwork_layer(n, 0)
+    full = circuits.build(specs, oracle.op)
+    reg = oracle.Reg(n, 0, threads=threads if threads > 1 else 1)
+    done, elapsed, count = 0, 0.0, 1
+    while elapsed < budget_s and done < len(full.gates):
+        count = min(count, len(full.gates) - done)
+        part = oracle.Op(full.gates[done:done + count])
+        t0 = time.perf_counter()
+        reg.apply(part)
+        dt = time.perf_counter() - t0
+        elapsed += dt
+        done += count
+        per = dt / count
+        count = max(1, int((budget_s - elapsed) / per / 2)) if per > 0 else 1
+    del reg
+    return {"value": done / elapsed, "unit": "gates/s", "cores": threads, "kind": "port",
+            "sample": "first %d SingleOps of brickwork layer 0 at %d qubits (FP64, one MultiOp::apply per batch, "
+                      "out-of-place gather + per-apply copy), %.1f s" % (done, n, elapsed)}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(kernel, {}).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import qvnt_amd
+    from qvnt_amd import QReg, circuits, op
+
+    n = args.qubits
+    layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]
+    ops = [circuits.build(s, op) for s in layers]
+    timed_gates = sum(len(o) for o in ops[args.warmup:])
+
+    reg = QReg(n)
+    log("register: %d qubits FP64, %.1f GiB" % (n, (16 << n) / 2**30))
+    for o in ops[:args.warmup]:
+        reg.apply(o)
+    reg.sync()
+    reg.set_timing(True)
+    reg.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for o in ops[args.warmup:]:
+        reg.apply(o)
+    reg.sync()
+    t1 = time.perf_counter()
+    stats = reg.kernel_stats(reset=True)
+    reg.set_timing(False)
+    norm = reg.get_absolute()
+    reg.close()
+    wall = t1 - t0
+    value = timed_gates / wall
+    log("plain: %d SingleOps in %.3f s -> %.1f gates/s; |psi|^2 = %.15f" % (timed_gates, wall, value, norm))
+    for k, s in stats.items():
+        if s["launches"]:
+            log("  %-8s %6d launches %10.2f ms  %8.1f GB/s" % (k, s["launches"], s["ms"], s["bytes"] / s["ms"] / 1e6))
+
+    dom = max((k for k in stats if stats[k]["launches"]), key=lambda k: stats[k]["ms"])
+    d = stats[dom]
+    per_launch_bytes = d["bytes"] / d["launches"]
+    avg_ms = d["ms"] / d["launches"]
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    kernel_name = {"diag": "k_diag", "pair": "k_pair", "quad": "k_quad"}.get(dom, dom)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kernel_name),
+                "kernel": kernel_name, "launches": d["launches"], "avg_ms": round(avg_ms, 4),
+                "bytes_per_launch": per_launch_bytes}
+    all_ms = sum(s["ms"] for s in stats.values())
+    all_bytes = sum(s["bytes"] for s in stats.values())
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, host cores) ...")
+        cpu = cpu_baseline(n, args.cpu_seconds)
+        log("cpu: %.3f gates/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "gates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded brickwork circuit, angles U[0,2pi))",
+        "config": {"workload": "brickwork-%dq-fp64-depth%d-plain" % (n, args.steps), "qubits": n,
+                   "state_gib": (16 << n) / 2**30, "singleops_timed": timed_gates,
+                   "form": "plain (one in-place kernel per SingleOp)"},
+        "roofline": roofline,
+        "all_kernels_gbs": round(all_bytes / all_ms / 1e6, 1) if all_ms else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+/*
+ * qvnt.h — C ABI of the MI355X state-vector engine (drop-in backend for the
+ * qvnt 0.4.4 register / operator API).
+ *
+ * The reference is a Rust crate; its public surface on this path is
+ *   op::{id,x,y,z,s,t,rx,ry,rz,rxx,ryy,rzz,swap,sqrt_swap,i_swap,sqrt_i_swap,
+ *        h,u1,u2,u3,qft,qft_swapped}          src/operator/mod.rs:100-522
+ *   Applicable::{c, dgr, act_on}              src/operator/applicable.rs:5-47
+ *   MultiOp (queue, `*` appends, Debug list)  src/operator/multi/mod.rs:63-191
+ *   QReg::{new, with_state, apply, get_probabilities, get_absolute,
+ *          measure_mask, measure}             src/register/quant.rs:113-507
+ * Every entry point below names the reference item it replaces.  A Rust
+ * `-sys` crate binds these symbols one-to-one.
+ *
+ * Conventions
+ *   - Every function returning `int` returns QVNT_OK (0) or a negative
+ *     QVNT_E_* code; qvnt_last_error() then holds a message (thread-local).
+ *   - Masks are uint64_t: bit k of a mask is qubit k is bit k of the
+ *     amplitude index (little-endian), as in the reference.
+ *   - Amplitudes are interleaved complex {re, im}: double for QVNT_F64,
+ *     float for QVNT_F32.  A register of n qubits stores max(2^n, 8)
+ *     amplitudes (src/register/quant.rs:15,116).
+ *   - A register handle owns its device memory and one HIP stream; it is not
+ *     thread-safe.  Distinct handles may be driven from different threads.
+ *   - Operator handles are host objects (gate metadata only, as in the
+ *     reference, src/operator/mod.rs:7-9).
+ */
+#ifndef QVNT_H
+#define QVNT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define QVNT_OK              0
+#define QVNT_E_INVALID_MASK (-1) /* wrong bit count: reference panics via expect
+                                    (src/operator/mod.rs:153,172,...)        */
+#define QVNT_E_CTRL_OVERLAP (-2) /* .c() overlaps act|ctrl: reference returns
+                                    None (src/operator/single/mod.rs:109-118) */
+#define QVNT_E_OOM          (-3) /* device memory / budget exhausted          */
+#define QVNT_E_HIP          (-4) /* HIP runtime error                         */
+#define QVNT_E_ZERO_NORM    (-5) /* measure on an all-zero/NaN state: reference
+                                    panics in WeightedIndex (quant.rs:497)    */
+#define QVNT_E_ARG          (-6) /* bad argument (null, size, mask beyond the
+                                    register: reference index panic)          */
+#define QVNT_E_NO_DEVICE    (-7) /* no usable gfx950 device                   */
+
+/* ------------------------------------------------------------ gate kinds
+ * One kind per reference atomic op (enum AtomicOpDispatch,
+ * src/operator/atomic/dispatch.rs:82-105), same order, plus PHI, which the
+ * reference README lists (README.md:69) but does not implement. */
+enum qvnt_kind {
+    QVNT_ID = 0,       /* atomic/id.rs           */
+    QVNT_X,            /* atomic/x.rs            */
+    QVNT_RX,           /* atomic/rx.rs           */
+    QVNT_RXX,          /* atomic/rxx.rs          */
+    QVNT_Y,            /* atomic/y.rs            */
+    QVNT_RY,           /* atomic/ry.rs           */
+    QVNT_RYY,          /* atomic/ryy.rs          */
+    QVNT_Z,            /* atomic/z.rs            */
+    QVNT_S,            /* atomic/s.rs            */
+    QVNT_T,            /* atomic/t.rs            */
+    QVNT_RZ,           /* atomic/rz.rs           */
+    QVNT_RZZ,          /* atomic/rzz.rs          */
+    QVNT_U1,           /* atomic/u1.rs (2x2 matrix on one qubit)  */
+    QVNT_U2,           /* atomic/u2.rs (4x4 matrix on two qubits) */
+    QVNT_H1,           /* atomic/h1.rs           */
+    QVNT_H2,           /* atomic/h2.rs           */
+    QVNT_SWAP,         /* atomic/swap.rs         */
+    QVNT_ISWAP,        /* atomic/i_swap.rs       */
+    QVNT_SQRT_SWAP,    /* atomic/sqrt_swap.rs    */
+    QVNT_SQRT_ISWAP,   /* atomic/sqrt_i_swap.rs  */
+    QVNT_PHI,          /* diag(1, e^{i theta}) on one qubit (new) */
+    QVNT_KIND_COUNT
+};
+
+/* One gate == one reference SingleOp {act, ctrl, func}
+ * (src/operator/single/mod.rs:42-47).  Plain old data, 312 bytes. */
+typedef struct qvnt_gate {
+    uint32_t kind;    /* enum qvnt_kind                                       */
+    uint32_t dagger;  /* S, T, ISWAP, SQRT_SWAP, SQRT_ISWAP: 1 = daggered     */
+    uint64_t act;     /* SingleOp::act (= AtomicOp::acts_on)                  */
+    uint64_t a;       /* H2 / U2: a_mask; other kinds: == act                 */
+    uint64_t b;       /* H2 / U2: b_mask; other kinds: 0                      */
+    uint64_t ctrl;    /* SingleOp::ctrl                                       */
+    double   re, im;  /* rotations: phase = (cos th/2, sin th/2) as the
+                         reference stores it; PHI: (cos th, sin th)           */
+    double   m[32];   /* U1: 4 complex (row-major 2x2); U2: 16 complex (4x4)  */
+} qvnt_gate;
+
+/* Amplitude precision of a register. */
+typedef enum qvnt_prec { QVNT_F64 = 0, QVNT_F32 = 1 } qvnt_prec;
+
+/* ------------------------------------------------------- errors / misc */
+const char* qvnt_last_error(void);
+/* Library version string, e.g. "qvnt-mi355x 0.1.0 (gfx950)". */
+const char* qvnt_version(void);
+
+/* Seeded uniform draw in [0, 1) used by measurement (the reference draws from
+ * an unseeded thread_rng, src/register/quant.rs:496-497, so a seeded contract
+ * replaces it): SplitMix64 of (seed, stream, index), top 53 bits * 2^-53. */
+double qvnt_uniform(uint64_t seed, uint64_t stream, uint64_t index);
+
+/* ------------------------------------------------------------ operators
+ * qvnt_op == reference MultiOp (a queue of SingleOps).  Constructors take the
+ * reference argument order (phase first, then mask).  Rotation and swap
+ * constructors return QVNT_E_INVALID_MASK where the reference panics. */
+typedef struct qvnt_op qvnt_op;
+
+int  qvnt_op_id(qvnt_op** out);                                   /* op::id           mod.rs:113 */
+int  qvnt_op_x(uint64_t a_mask, qvnt_op** out);                   /* op::x            mod.rs:136 */
+int  qvnt_op_y(uint64_t a_mask, qvnt_op** out);                   /* op::y            mod.rs:199 */
+int  qvnt_op_z(uint64_t a_mask, qvnt_op** out);                   /* op::z            mod.rs:258 */
+int  qvnt_op_s(uint64_t a_mask, qvnt_op** out);                   /* op::s            mod.rs:277 */
+int  qvnt_op_t(uint64_t a_mask, qvnt_op** out);                   /* op::t            mod.rs:297 */
+int  qvnt_op_rx(double phase, uint64_t a_mask, qvnt_op** out);    /* op::rx           mod.rs:151 */
+int  qvnt_op_ry(double phase, uint64_t a_mask, qvnt_op** out);    /* op::ry           mod.rs:214 */
+int  qvnt_op_rz(double phase, uint64_t a_mask, qvnt_op** out);    /* op::rz           mod.rs:321 */
+int  qvnt_op_rxx(double phase, uint64_t ab_mask, qvnt_op** out);  /* op::rxx          mod.rs:170 */
+int  qvnt_op_ryy(double phase, uint64_t ab_mask, qvnt_op** out);  /* op::ryy          mod.rs:233 */
+int  qvnt_op_rzz(double phase, uint64_t ab_mask, qvnt_op** out);  /* op::rzz          mod.rs:340 */
+int  qvnt_op_phi(double phase, uint64_t a_mask, qvnt_op** out);   /* README.md:69 "phi" (absent in code) */
+int  qvnt_op_swap(uint64_t ab_mask, qvnt_op** out);               /* op::swap         mod.rs:365 */
+int  qvnt_op_sqrt_swap(uint64_t ab_mask, qvnt_op** out);          /* op::sqrt_swap    mod.rs:387 */
+int  qvnt_op_i_swap(uint64_t ab_mask, qvnt_op** out);             /* op::i_swap       mod.rs:412 */
+int  qvnt_op_sqrt_i_swap(uint64_t ab_mask, qvnt_op** out);        /* op::sqrt_i_swap  mod.rs:434 */
+int  qvnt_op_h(uint64_t a_mask, qvnt_op** out);                   /* op::h            mod.rs:464, multi/h.rs:14-45 */
+int  qvnt_op_u1(double lam, uint64_t a_mask, qvnt_op** out);      /* op::u1           mod.rs:472-474 */
+int  qvnt_op_u2(double phi, double lam, uint64_t a_mask, qvnt_op** out);             /* op::u2 mod.rs:480-482 */
+int  qvnt_op_u3(double the, double phi, double lam, uint64_t a_mask, qvnt_op** out); /* op::u3 mod.rs:499-501 */
+int  qvnt_op_qft(uint64_t a_mask, qvnt_op** out);                 /* op::qft          mod.rs:511-513, multi/qft.rs:4-33 */
+int  qvnt_op_qft_swapped(uint64_t a_mask, qvnt_op** out);         /* op::qft_swapped  mod.rs:520, multi/qft.rs:35-52 */
+/* General-matrix atomics (atomic/u1.rs, atomic/u2.rs; crate-internal in the
+ * reference, single/pauli.rs:32-39).  QVNT_E_INVALID_MASK unless the mask bit
+ * counts are right and the matrix is unitary to 1e-12. */
+int  qvnt_op_matrix1(uint64_t a_mask, const double m[8], qvnt_op** out);
+int  qvnt_op_matrix2(uint64_t a_mask, uint64_t b_mask, const double m[32], qvnt_op** out);
+/* An operator from raw gates (copied; "Id" gates are dropped as in
+ * From<SingleOp> for MultiOp, multi/mod.rs:135-146). */
+int  qvnt_op_from_gates(const qvnt_gate* gates, size_t count, qvnt_op** out);
+int  qvnt_op_clone(const qvnt_op* op, qvnt_op** out);             /* Clone */
+void qvnt_op_free(qvnt_op* op);
+/* lhs *= rhs: appends rhs's queue (MulAssign, multi/mod.rs:166-170). */
+int  qvnt_op_mul(qvnt_op* lhs, const qvnt_op* rhs);
+/* Applicable::c in place (multi/mod.rs:125-132): QVNT_E_CTRL_OVERLAP (the
+ * reference's None) leaves the operator unchanged. */
+int  qvnt_op_c(qvnt_op* op, uint64_t c_mask);
+/* Applicable::dgr in place: reverse the queue, dgr each (multi/mod.rs:120-123). */
+int  qvnt_op_dgr(qvnt_op* op);
+uint64_t qvnt_op_act_on(const qvnt_op* op);                       /* multi/mod.rs:116-118 */
+size_t   qvnt_op_len(const qvnt_op* op);                          /* VecDeque::len */
+/* Borrowed view of the gate queue, valid until the op is changed or freed. */
+const qvnt_gate* qvnt_op_gates(const qvnt_op* op);
+/* Debug string of the MultiOp, e.g. "[H3, H12, C8_H3, C2_SWAP9]"
+ * (multi/mod.rs:90-94, single/mod.rs:70-76).  Writes at most cap bytes
+ * (NUL-terminated); *need gets the full length + 1. */
+int  qvnt_op_name(const qvnt_op* op, char* buf, size_t cap, size_t* need);
+
+/* ------------------------------------------------------------- registers */
+typedef struct qvnt_reg qvnt_reg;
+
+/* QReg::new (quant.rs:113-125) == with_state(q_num, 0). */
+int qvnt_reg_new(uint32_t q_num, qvnt_prec prec, qvnt_reg** out);
+/* QReg::with_state (quant.rs:129-150): psi[state & q_mask] = 1. */
+int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg** out);
+int qvnt_reg_free(qvnt_reg* reg);
+uint32_t qvnt_reg_num(const qvnt_reg* reg);                       /* QReg::num */
+/* Number of stored amplitudes, max(2^n, 8). */
+uint64_t qvnt_reg_len(const qvnt_reg* reg);
+qvnt_prec qvnt_reg_prec(const qvnt_reg* reg);
+
+/* QReg::apply(&MultiOp) (quant.rs:376-395): the plain form — one in-place
+ * kernel per SingleOp, in queue order, on the register's stream.  Results
+ * equal the reference's out-of-place gather (dispatch.rs:32-47). */
+int qvnt_reg_apply(qvnt_reg* reg, const qvnt_op* op);
+/* Same for a raw gate array (borrowed for the call). */
+int qvnt_reg_apply_gates(qvnt_reg* reg, const qvnt_gate* gates, size_t count);
+
+/* QReg::get_absolute (quant.rs:458-466): sum of |psi_i|^2 over the buffer. */
+int qvnt_reg_get_absolute(qvnt_reg* reg, double* out);
+/* QReg::get_probabilities (quant.rs:434-454): p_i = |psi_i|^2 / total for
+ * i < 2^n.  dst must hold 2^n doubles. */
+int qvnt_reg_get_probabilities(qvnt_reg* reg, double* dst, uint64_t len);
+
+/* QReg::measure_mask (quant.rs:490-501).  u is the caller's seeded uniform
+ * draw in [0,1).  *out = r & mask where r is drawn with probability
+ * |psi_r|^2 / total, then every amplitude with (idx ^ r) & mask != 0 is
+ * zeroed.  flags & QVNT_MEASURE_RENORM rescales the kept amplitudes to unit
+ * norm in FP64 (the reference leaves norm^2 = P(outcome)). */
+#define QVNT_MEASURE_RENORM 1u
+int qvnt_reg_measure_mask(qvnt_reg* reg, uint64_t mask, double u, uint32_t flags, uint64_t* out);
+/* QReg::measure (quant.rs:505-507) == measure_mask(q_mask). */
+int qvnt_reg_measure(qvnt_reg* reg, double u, uint32_t flags, uint64_t* out);
+/* Non-destructive shots: out[k] = r_k & mask for draws u[k], state untouched
+ * (each equals measure_mask on a copy of the register). */
+int qvnt_reg_sample_mask(qvnt_reg* reg, uint64_t mask, const double* u, size_t shots, uint64_t* out);
+
+/* Copy-out / copy-in of the interleaved amplitudes (bytes = len * 2 *
+ * sizeof(double or float)).  Blocking. */
+int qvnt_reg_get_state(qvnt_reg* reg, void* dst, size_t bytes);
+int qvnt_reg_set_state(qvnt_reg* reg, const void* src, size_t bytes);
+/* Wait for all work queued on the register's stream. */
+int qvnt_reg_sync(qvnt_reg* reg);
+
+/* --------------------------------------------------------- instrumentation
+ * Per-kernel-class device timing with HIP events recorded on the register's
+ * stream around every launch (off by default). */
+enum qvnt_kclass {
+    QVNT_K_DIAG = 0,   /* phase-only diagonal, single pass                */
+    QVNT_K_PAIR,       /* 2-amplitude orbit (pair mix / permutation)      */
+    QVNT_K_QUAD,       /* 4-amplitude orbit (H2, U2)                      */
+    QVNT_K_FUSED,      /* fused multi-gate tile pass                      */
+    QVNT_K_REDUCE,     /* |psi|^2 tree reduction                          */
+    QVNT_K_COLLAPSE,   /* measurement collapse / rescale                  */
+    QVNT_K_OTHER,
+    QVNT_KCLASS_COUNT
+};
+typedef struct qvnt_kstats {
+    uint64_t launches;
+    double   ms;       /* summed event time                               */
+    double   bytes;    /* summed algorithmic bytes 2*s*2^(n-popcount(ctrl)) */
+} qvnt_kstats;
+int qvnt_reg_set_timing(qvnt_reg* reg, int enable);
+/* Synchronises, folds pending events into the totals, copies
+ * QVNT_KCLASS_COUNT entries into out, and (if reset) clears them. */
+int qvnt_reg_kernel_stats(qvnt_reg* reg, qvnt_kstats* out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QVNT_H */

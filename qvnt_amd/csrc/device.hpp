@@ -1,0 +1,169 @@
+// Device-side building blocks shared by the HIP translation units:
+// complex amplitudes, index deposit, and the per-gate formulas.
+//
+// The formulas restate the reference's AtomicOp::atomic_op of every gate
+// (src/operator/atomic/<gate>.rs) expression by expression, so that with
+// FP contraction off a double-precision run rounds exactly like the
+// reference's CPU loop.  num-complex's Mul is (a.re*b.re - a.im*b.im,
+// a.re*b.im + a.im*b.re) with `a` the left operand.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/qvnt.h"
+
+namespace qvnt {
+
+template <class R>
+struct alignas(2 * sizeof(R)) cplx {
+    R re, im;
+};
+using c64 = cplx<double>;
+using c32 = cplx<float>;
+
+template <class R>
+__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R>
+__device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> cneg(cplx<R> a) { return {-a.re, -a.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> cscale(cplx<R> a, R t) { return {a.re * t, a.im * t}; }
+// math::rotate (src/math/mod.rs:41-50): i^q * z, q mod 4.
+template <class R>
+__device__ __forceinline__ cplx<R> rotate(cplx<R> z, uint64_t q) {
+    if (q & 2u) z = cneg(z);
+    if (q & 1u) z = cplx<R>{-z.im, z.re};
+    return z;
+}
+__device__ __forceinline__ unsigned dpopc(uint64_t x) { return (unsigned)__popcll(x); }
+
+// Bit positions inserted into a compact thread index to enumerate one
+// representative per orbit: `pos` ascending, then `set` ORed in (control
+// bits, and a fixed 1 where the orbit needs one).
+struct Deposit {
+    uint32_t n;
+    uint32_t pad;
+    uint64_t set;
+    uint8_t pos[64];
+};
+
+__device__ __forceinline__ uint64_t deposit(uint64_t g, const Deposit& d) {
+    for (uint32_t k = 0; k < d.n; ++k) {
+        const uint64_t lo = g & ((1ull << d.pos[k]) - 1u);
+        g = ((g ^ lo) << 1) | lo;
+    }
+    return g | d.set;
+}
+
+// Gate parameters in the amplitude precision R.
+template <class R>
+struct Par {
+    uint64_t a, b;       // a: act mask (orbit mask for pair kinds); b: H2/U2 b_mask
+    R re, im;            // stored phase
+    R m[32];             // U1 / U2 matrices
+    uint32_t dagger;
+    uint32_t i_pow;      // Y: !(popcount(a) + 1) as usize (atomic/y.rs:11)
+};
+
+constexpr double kFrac1Sqrt2 = 0.70710678118654752440;
+
+// ------------------------------------------------------------- diagonals
+// out[idx] = f(psi[idx]); kinds Z, S, T, RZ, RZZ, PHI.
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_diag(const Par<R>& g, uint64_t idx, cplx<R> p) {
+    if constexpr (K == QVNT_Z) {                                   // z.rs:15-21
+        return (dpopc(idx & g.a) & 1u) ? cneg(p) : p;
+    } else if constexpr (K == QVNT_S) {                            // s.rs:19-25
+        uint64_t count = dpopc(idx & g.a);
+        if (g.dagger) count = (~count) + 1u;
+        return rotate(p, count);
+    } else if constexpr (K == QVNT_T) {                            // t.rs:24-35
+        uint64_t count = dpopc(idx & g.a);
+        if (g.dagger) count = (~count) + 1u;
+        const cplx<R> z = rotate(p, count >> 1);
+        const cplx<R> e{(R)kFrac1Sqrt2, (R)kFrac1Sqrt2};
+        return (count & 1u) ? cmul(e, z) : z;
+    } else if constexpr (K == QVNT_RZ) {                           // rz.rs:19-25
+        cplx<R> ph{g.re, g.im};
+        if ((idx & g.a) == 0) ph.im = -ph.im;
+        return cmul(ph, p);
+    } else if constexpr (K == QVNT_RZZ) {                          // rzz.rs:19-25
+        cplx<R> ph{g.re, g.im};
+        if ((dpopc(idx & g.a) & 1u) == 0) ph.im = -ph.im;
+        return cmul(ph, p);
+    } else {                                                       // PHI: diag(1, e^{i th})
+        const cplx<R> ph{g.re, g.im};
+        return (idx & g.a) ? cmul(ph, p) : p;
+    }
+}
+
+// ---------------------------------------------------------- pair orbits
+// out[idx] = f(psi[idx], psi[idx ^ m]); `self` = psi[idx],
+// `other` = psi[idx ^ m] with m the orbit mask.
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_pair(const Par<R>& g, uint64_t idx, cplx<R> self, cplx<R> other) {
+    if constexpr (K == QVNT_X) {                                   // x.rs:15-17
+        return other;
+    } else if constexpr (K == QVNT_Y) {                            // y.rs:17-23
+        uint64_t q = g.i_pow;
+        if ((dpopc(idx & g.a) & 1u) == 0) q ^= 2u;
+        return rotate(other, q);
+    } else if constexpr (K == QVNT_RX || K == QVNT_RXX) {          // rx.rs:18-24, rxx.rs:19-25
+        return {self.re * g.re + other.im * g.im, self.im * g.re - other.re * g.im};
+    } else if constexpr (K == QVNT_RY) {                           // ry.rs:19-29
+        R s = g.im;
+        if ((idx & g.a) == 0) s = -s;
+        return {self.re * g.re + other.re * s, self.im * g.re + other.im * s};
+    } else if constexpr (K == QVNT_RYY) {                          // ryy.rs:19-29
+        R s = g.im;
+        if ((dpopc(idx & g.a) & 1u) == 0) s = -s;
+        return {self.re * g.re + other.im * s, self.im * g.re - other.re * s};
+    } else if constexpr (K == QVNT_H1) {                           // h1.rs:16-22
+        cplx<R> p0 = self;
+        if (idx & g.a) p0 = cneg(p0);
+        return cscale(cadd(p0, other), (R)kFrac1Sqrt2);
+    } else if constexpr (K == QVNT_U1) {                           // u1.rs:17-25
+        if ((idx & g.a) == 0)
+            return cadd(cmul(cplx<R>{g.m[0], g.m[1]}, self), cmul(cplx<R>{g.m[2], g.m[3]}, other));
+        return cadd(cmul(cplx<R>{g.m[4], g.m[5]}, other), cmul(cplx<R>{g.m[6], g.m[7]}, self));
+    } else if constexpr (K == QVNT_SWAP) {                         // swap.rs:16-22 (odd orbits)
+        return other;
+    } else if constexpr (K == QVNT_ISWAP) {                        // i_swap.rs:20-37 (odd orbits)
+        if (g.dagger) return {other.im, -other.re};
+        return {-other.im, other.re};
+    } else if constexpr (K == QVNT_SQRT_SWAP) {                    // sqrt_swap.rs:20-37 (odd orbits)
+        const cplx<R> p0 = self, p1 = other;
+        if (g.dagger)
+            return {(R)0.5 * (p0.re + p0.im + p1.re - p1.im), (R)0.5 * (p0.im - p0.re + p1.im + p1.re)};
+        return {(R)0.5 * (p0.re - p0.im + p1.re + p1.im), (R)0.5 * (p0.im + p0.re + p1.im - p1.re)};
+    } else {                                                       // sqrt_i_swap.rs:20-37 (odd orbits)
+        const cplx<R> p0 = self, p1 = other;
+        if (g.dagger) return {(R)kFrac1Sqrt2 * (p0.re + p1.im), (R)kFrac1Sqrt2 * (p0.im - p1.re)};
+        return {(R)kFrac1Sqrt2 * (p0.re - p1.im), (R)kFrac1Sqrt2 * (p0.im + p1.re)};
+    }
+}
+
+// ---------------------------------------------------------- quad orbits
+// v[k] = psi[base | sel_k] with sel bits (b, a) = k; output for v-slot `s`.
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_quad(const Par<R>& g, int s, const cplx<R>* v) {
+    if constexpr (K == QVNT_H2) {                                  // h2.rs:22-38
+        cplx<R> t0 = v[s], t1 = v[s ^ 1], t2 = v[s ^ 2], t3 = v[s ^ 3];
+        if (s & 1) { t0 = cneg(t0); t2 = cneg(t2); }
+        if (s & 2) { t0 = cneg(t0); t1 = cneg(t1); }
+        return cscale(cadd(cadd(cadd(t0, t1), t2), t3), (R)0.5);
+    } else {                                                       // u2.rs:22-49
+        const R* m = g.m + 8 * s;
+        cplx<R> acc = cmul(cplx<R>{m[0], m[1]}, v[0]);
+        acc = cadd(acc, cmul(cplx<R>{m[2], m[3]}, v[1]));
+        acc = cadd(acc, cmul(cplx<R>{m[4], m[5]}, v[2]));
+        acc = cadd(acc, cmul(cplx<R>{m[6], m[7]}, v[3]));
+        return acc;
+    }
+}
+
+}  // namespace qvnt

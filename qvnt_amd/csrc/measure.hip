@@ -1,0 +1,339 @@
+// Measurement: QReg::get_absolute / get_probabilities / measure_mask
+// (src/register/quant.rs:434-507) on the device.
+//
+// The reference sums |psi|^2 sequentially, builds rand's WeightedIndex
+// cumulative array on the host thread and draws from an unseeded thread_rng
+// (quant.rs:437-442, 496-497).  Its draw cannot be reproduced, so a
+// deterministic contract replaces it (the CPU oracle implements the same):
+//   w_i = re*re + im*im in FP64 (no FMA), node(0,j) = w_j,
+//   node(l,j) = node(l-1,2j) + node(l-1,2j+1)        (pairwise tree),
+//   draw: x = u * node(top,0); from the root, with children (L, R), go left
+//   iff x < L or R == 0, else x -= L and go right; the leaf reached is r.
+// Tree levels from 2^11 leaves upwards are stored (O(2^n / 2^11) doubles);
+// a draw recomputes the 2^11-leaf subtree it lands in.  FP32 registers are
+// reduced, searched and rescaled in FP64 as well.
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <vector>
+
+#include "device.hpp"
+#include "reg.hpp"
+
+namespace qvnt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr unsigned kLeafLog = 11;          // leaves per stored bottom node
+constexpr unsigned kLeaf = 1u << kLeafLog;
+
+template <class R>
+__device__ __forceinline__ double weight(cplx<R> p) {
+    const double re = (double)p.re, im = (double)p.im;
+    return re * re + im * im;
+}
+
+// node(11, blockIdx.x): each wave reduces 512 consecutive leaves, 64 per
+// load; lanes pair up with xor-shuffles (adjacent pairs first), then the 8
+// chunk sums and the 4 wave sums pair up in the same adjacent order.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_leaf(const cplx<R>* __restrict__ psi, double* __restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * kLeaf + (uint64_t)wave * 512u;
+    double s[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = weight(psi[base + (uint64_t)j * 64u + (uint64_t)lane]);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] = s[j] + __shfl_xor(s[j], o);
+    const double w = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    __shared__ double sh[4];
+    if (lane == 0) sh[wave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// Pairwise levels of `span` (<= 2048, a power of two) values held in LDS:
+// lv + off(k) with off(k) = 2*span - (2*span >> k).
+__device__ void lds_levels(double* lv, unsigned span) {
+    unsigned in_off = 0, m = span;
+    while (m > 1) {
+        const unsigned h = m >> 1, out_off = in_off + m;
+        for (unsigned t = threadIdx.x; t < h; t += kBlock) lv[out_off + t] = lv[in_off + 2 * t] + lv[in_off + 2 * t + 1];
+        __syncthreads();
+        in_off = out_off;
+        m = h;
+    }
+}
+
+// Whole tree of a buffer of <= 2048 leaves: writes node(top, 0).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_small(const cplx<R>* __restrict__ psi, unsigned count, double* __restrict__ out) {
+    __shared__ double lv[2 * kLeaf];
+    for (unsigned t = threadIdx.x; t < count; t += kBlock) lv[t] = weight(psi[t]);
+    __syncthreads();
+    lds_levels(lv, count);
+    if (threadIdx.x == 0) out[0] = lv[2 * count - 2];
+}
+
+struct LevelPtrs {
+    double* p[kLeafLog + 1];
+};
+
+// Levels L+1 .. L+log2(span) above a stored level L of n_in nodes; every
+// block owns `span` consecutive nodes.
+__global__ __launch_bounds__(kBlock) void k_upper(const double* __restrict__ in, unsigned span, LevelPtrs out) {
+    __shared__ double lv[2 * kLeaf];
+    const uint64_t base = (uint64_t)blockIdx.x * span;
+    for (unsigned t = threadIdx.x; t < span; t += kBlock) lv[t] = in[base + t];
+    __syncthreads();
+    lds_levels(lv, span);
+    unsigned off = span, m = span >> 1;
+    for (unsigned k = 1; m >= 1; ++k, off += m, m >>= 1)
+        for (unsigned t = threadIdx.x; t < m; t += kBlock) out.p[k][(uint64_t)blockIdx.x * m + t] = lv[off + t];
+}
+
+struct TreeInfo {
+    uint32_t top;      // log2(count)
+    uint32_t lb;       // lowest stored level, min(top, 11)
+    uint64_t off[64];  // offset of stored level l (lb <= l <= top) in the tree array
+};
+
+// One draw per block: descend the stored levels, then the recomputed
+// 2^lb-leaf subtree of the block reached.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_sample(const cplx<R>* __restrict__ psi, const double* __restrict__ tree,
+                                                   TreeInfo info, const double* __restrict__ u, uint64_t* __restrict__ out,
+                                                   double* __restrict__ total) {
+    __shared__ double lv[2 * kLeaf];
+    __shared__ uint64_t sj;
+    __shared__ double sx;
+    if (threadIdx.x == 0) {
+        const double T = tree[info.off[info.top]];
+        double x = u[blockIdx.x] * T;
+        uint64_t j = 0;
+        for (uint32_t l = info.top; l > info.lb; --l) {
+            const double L = tree[info.off[l - 1] + 2 * j], Rr = tree[info.off[l - 1] + 2 * j + 1];
+            if (x < L || Rr == 0.0) {
+                j = 2 * j;
+            } else {
+                x = x - L;
+                j = 2 * j + 1;
+            }
+        }
+        sj = j;
+        sx = x;
+        if (blockIdx.x == 0) total[0] = T;
+    }
+    __syncthreads();
+    const unsigned span = 1u << info.lb;
+    const uint64_t base = sj << info.lb;
+    for (unsigned t = threadIdx.x; t < span; t += kBlock) lv[t] = weight(psi[base + t]);
+    __syncthreads();
+    lds_levels(lv, span);
+    if (threadIdx.x == 0) {
+        double x = sx;
+        uint64_t k = 0;
+        // level l of the block subtree lives at off(l) = 2*span - (2*span >> l)
+        for (uint32_t l = info.lb; l > 0; --l) {
+            const unsigned off = 2 * span - ((2 * span) >> (l - 1));
+            const double L = lv[off + 2 * k], Rr = lv[off + 2 * k + 1];
+            if (x < L || Rr == 0.0) {
+                k = 2 * k;
+            } else {
+                x = x - L;
+                k = 2 * k + 1;
+            }
+        }
+        out[blockIdx.x] = base + k;
+    }
+}
+
+// collapse_mask (quant.rs:468-486): zero every amplitude with
+// (idx ^ r) & mask != 0; store-only.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_collapse(cplx<R>* __restrict__ psi, uint64_t len, uint64_t r, uint64_t mask) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len && ((i ^ r) & mask)) psi[i] = cplx<R>{(R)0, (R)0};
+}
+
+// Renormalisation of the kept subspace (the bits of `mask` fixed to r's),
+// multiplied in FP64.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_rescale(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, double scale) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= groups) return;
+    const uint64_t i = deposit(t, dep);
+    const cplx<R> p = psi[i];
+    psi[i] = cplx<R>{(R)((double)p.re * scale), (R)((double)p.im * scale)};
+}
+
+// get_probabilities (quant.rs:434-454): p_i = |psi_i|^2 * inv.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_probs(const cplx<R>* __restrict__ psi, uint64_t n, double inv, double* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = weight(psi[i]) * inv;
+}
+
+unsigned log2u(uint64_t x) { return 63u - (unsigned)__builtin_clzll(x); }
+
+TreeInfo tree_info(uint64_t count) {
+    TreeInfo t{};
+    t.top = log2u(count);
+    t.lb = t.top < kLeafLog ? t.top : kLeafLog;
+    uint64_t off = 0;
+    for (uint32_t l = t.lb; l <= t.top; ++l) {
+        t.off[l] = off;
+        off += count >> l;
+    }
+    return t;
+}
+
+int ensure_workspace(qvnt_reg* r, size_t tree_doubles, size_t shots) {
+    if (tree_doubles > r->tree_cap) {
+        if (r->tree) QVNT_HIP(hipFree(r->tree));
+        r->tree = nullptr;
+        r->tree_cap = 0;
+        if (hipMalloc(&r->tree, tree_doubles * sizeof(double)) != hipSuccess)
+            return fail(QVNT_E_OOM, "device memory for the measurement tree");
+        r->tree_cap = tree_doubles;
+    }
+    if (shots > r->shots_cap || !r->dscratch) {
+        const size_t cap = shots < 64 ? 64 : shots;
+        if (r->dscratch) QVNT_HIP(hipFree(r->dscratch));
+        if (r->iscratch) QVNT_HIP(hipFree(r->iscratch));
+        r->dscratch = nullptr;
+        r->iscratch = nullptr;
+        if (hipMalloc(&r->dscratch, (cap + 1) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&r->iscratch, cap * sizeof(uint64_t)) != hipSuccess)
+            return fail(QVNT_E_OOM, "device memory for measurement scratch");
+        r->shots_cap = cap;
+    }
+    return QVNT_OK;
+}
+
+template <class R>
+int build_levels(qvnt_reg* r, uint64_t count, const TreeInfo& info) {
+    const cplx<R>* psi = static_cast<const cplx<R>*>(r->psi);
+    Timed t(r, QVNT_K_REDUCE, (double)r->amp_bytes() * (double)count);
+    if (info.top < kLeafLog) {
+        hipLaunchKernelGGL(k_small<R>, dim3(1), dim3(kBlock), 0, r->stream, psi, (unsigned)count, r->tree + info.off[info.top]);
+        QVNT_HIP(hipGetLastError());
+        return QVNT_OK;
+    }
+    hipLaunchKernelGGL(k_leaf<R>, dim3((unsigned)(count >> kLeafLog)), dim3(kBlock), 0, r->stream, psi,
+                       r->tree + info.off[kLeafLog]);
+    QVNT_HIP(hipGetLastError());
+    for (uint32_t L = kLeafLog; L < info.top;) {
+        const uint32_t steps = (info.top - L) < kLeafLog ? (info.top - L) : kLeafLog;
+        const unsigned span = 1u << steps;
+        const uint64_t n_in = count >> L;
+        LevelPtrs lp{};
+        for (uint32_t k = 1; k <= steps; ++k) lp.p[k] = r->tree + info.off[L + k];
+        hipLaunchKernelGGL(k_upper, dim3((unsigned)(n_in / span)), dim3(kBlock), 0, r->stream,
+                           (const double*)(r->tree + info.off[L]), span, lp);
+        QVNT_HIP(hipGetLastError());
+        L += steps;
+    }
+    return QVNT_OK;
+}
+
+}  // namespace
+
+int tree_build(qvnt_reg* r, uint64_t count, double* total_out) {
+    const TreeInfo info = tree_info(count);
+    int rc = ensure_workspace(r, (size_t)(info.off[info.top] + 1), 1);
+    if (rc) return rc;
+    rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
+    if (rc) return rc;
+    if (total_out) {
+        QVNT_HIP(hipMemcpyAsync(total_out, r->tree + info.off[info.top], sizeof(double), hipMemcpyDeviceToHost, r->stream));
+        QVNT_HIP(hipStreamSynchronize(r->stream));
+    }
+    return QVNT_OK;
+}
+
+int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out) {
+    const TreeInfo info = tree_info(count);
+    int rc = ensure_workspace(r, (size_t)(info.off[info.top] + 1), shots);
+    if (rc) return rc;
+    rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
+    if (rc) return rc;
+    QVNT_HIP(hipMemcpyAsync(r->dscratch + 1, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    {
+        Timed t(r, QVNT_K_OTHER, 0.0);
+        if (r->prec == QVNT_F64)
+            hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                               static_cast<const c64*>(r->psi), (const double*)r->tree, info,
+                               (const double*)(r->dscratch + 1), r->iscratch, r->dscratch);
+        else
+            hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                               static_cast<const c32*>(r->psi), (const double*)r->tree, info,
+                               (const double*)(r->dscratch + 1), r->iscratch, r->dscratch);
+        QVNT_HIP(hipGetLastError());
+    }
+    QVNT_HIP(hipMemcpyAsync(out, r->iscratch, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
+    QVNT_HIP(hipMemcpyAsync(total_out, r->dscratch, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    return QVNT_OK;
+}
+
+int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm) {
+    {
+        const dim3 grid((unsigned)((r->len + kBlock - 1) / kBlock));
+        Timed t(r, QVNT_K_COLLAPSE, 2.0 * (double)r->amp_bytes() * (double)r->len);
+        if (r->prec == QVNT_F64)
+            hipLaunchKernelGGL(k_collapse<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), r->len, outcome, mask);
+        else
+            hipLaunchKernelGGL(k_collapse<float>, grid, dim3(kBlock), 0, r->stream, static_cast<c32*>(r->psi), r->len, outcome, mask);
+        QVNT_HIP(hipGetLastError());
+    }
+    if (!renorm) return QVNT_OK;
+    double kept = 0.0;
+    int rc = tree_build(r, r->len, &kept);
+    if (rc) return rc;
+    if (!(kept > 0.0) || !std::isfinite(kept)) return fail(QVNT_E_ZERO_NORM, "collapsed state has zero norm");
+    const double scale = 1.0 / std::sqrt(kept);
+    Deposit d{};
+    for (uint64_t rest = mask; rest; rest &= rest - 1) {
+        const unsigned p = ctz(rest);
+        unsigned k = d.n++;
+        while (k > 0 && d.pos[k - 1] > p) { d.pos[k] = d.pos[k - 1]; --k; }
+        d.pos[k] = (uint8_t)p;
+    }
+    d.set = outcome & mask;
+    const uint64_t groups = r->len >> d.n;
+    const dim3 grid((unsigned)((groups + kBlock - 1) / kBlock));
+    Timed t(r, QVNT_K_COLLAPSE, 2.0 * (double)r->amp_bytes() * (double)groups);
+    if (r->prec == QVNT_F64)
+        hipLaunchKernelGGL(k_rescale<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), groups, d, scale);
+    else
+        hipLaunchKernelGGL(k_rescale<float>, grid, dim3(kBlock), 0, r->stream, static_cast<c32*>(r->psi), groups, d, scale);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n) {
+    // bounded device staging: chunks of up to 2^24 probabilities
+    const uint64_t chunk = n < (1ull << 24) ? n : (1ull << 24);
+    double* buf = nullptr;
+    if (hipMalloc(&buf, chunk * sizeof(double)) != hipSuccess) return fail(QVNT_E_OOM, "probability staging buffer");
+    int rc = QVNT_OK;
+    for (uint64_t off = 0; off < n && rc == QVNT_OK; off += chunk) {
+        const uint64_t m = (n - off) < chunk ? (n - off) : chunk;
+        const dim3 grid((unsigned)((m + kBlock - 1) / kBlock));
+        if (r->prec == QVNT_F64)
+            hipLaunchKernelGGL(k_probs<double>, grid, dim3(kBlock), 0, r->stream, static_cast<const c64*>(r->psi) + off, m, inv_total, buf);
+        else
+            hipLaunchKernelGGL(k_probs<float>, grid, dim3(kBlock), 0, r->stream, static_cast<const c32*>(r->psi) + off, m, inv_total, buf);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(dst + off, buf, m * sizeof(double), hipMemcpyDeviceToHost, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) rc = fail(QVNT_E_HIP, std::string("probabilities: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(buf);
+    return rc;
+}
+
+}  // namespace qvnt

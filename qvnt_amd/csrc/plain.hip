@@ -1,0 +1,284 @@
+// The plain form of QReg::apply: one in-place kernel per SingleOp.
+//
+// The reference gathers every output element from a read-only input buffer
+// (src/operator/atomic/dispatch.rs:32-47, multi/mod.rs:96-104), which needs
+// a second state-sized buffer and streams the whole state even for
+// controlled gates.  Here each thread owns whole index orbits of the gate
+// (1, 2 or 4 amplitudes that the gate maps onto each other), reads them,
+// and writes them back in place:
+//   - only the control-satisfied subspace is enumerated: control bits are
+//     deposited as fixed 1s into the compact thread index, so no lane is
+//     wasted on a failed predicate;
+//   - orbits that the gate leaves unchanged (even-parity pairs of the swap
+//     family, the |0> half of single-bit phase gates) are not touched;
+//   - consecutive threads get consecutive free index bits, so every wave
+//     load is one or two contiguous runs whatever the target qubit.
+// FP contraction is off in this file: outputs equal the reference's to the
+// last bit in FP64.
+#pragma clang fp contract(off)
+
+#include <cstdlib>
+
+#include "device.hpp"
+#include "reg.hpp"
+
+namespace qvnt {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <int K, class R, int U>
+__global__ __launch_bounds__(kBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+    cplx<R> v[U];
+    uint64_t idx[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        idx[u] = deposit(t, dep);
+        if (t < groups) v[u] = psi[idx[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        if (t < groups) psi[idx[u]] = f_diag<K>(g, idx[u], v[u]);
+    }
+}
+
+template <int K, class R, int U>
+__global__ __launch_bounds__(kBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+    cplx<R> v0[U], v1[U];
+    uint64_t i0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        i0[u] = deposit(t, dep);
+        if (t < groups) {
+            v0[u] = psi[i0[u]];
+            v1[u] = psi[i0[u] ^ g.a];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        if (t < groups) {
+            const uint64_t i1 = i0[u] ^ g.a;
+            const cplx<R> o0 = f_pair<K>(g, i0[u], v0[u], v1[u]);
+            const cplx<R> o1 = f_pair<K>(g, i1, v1[u], v0[u]);
+            psi[i0[u]] = o0;
+            psi[i1] = o1;
+        }
+    }
+}
+
+template <int K, class R, int U>
+__global__ __launch_bounds__(kBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+    cplx<R> v[U][4];
+    uint64_t i0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        i0[u] = deposit(t, dep);
+        if (t < groups) {
+            v[u][0] = psi[i0[u]];
+            v[u][1] = psi[i0[u] | g.a];
+            v[u][2] = psi[i0[u] | g.b];
+            v[u][3] = psi[i0[u] | g.a | g.b];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * kBlock;
+        if (t < groups) {
+            cplx<R> o[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) o[s] = f_quad<K>(g, s, v[u]);
+            psi[i0[u]] = o[0];
+            psi[i0[u] | g.a] = o[1];
+            psi[i0[u] | g.b] = o[2];
+            psi[i0[u] | g.a | g.b] = o[3];
+        }
+    }
+}
+
+// QReg::with_state (quant.rs:129-150): zeros, psi[state] = 1.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_init(cplx<R>* __restrict__ psi, uint64_t len, uint64_t state) {
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) psi[i] = cplx<R>{i == state ? (R)1 : (R)0, (R)0};
+}
+
+// ----------------------------------------------------------- host side
+int unroll_setting() {
+    static int u = [] {
+        const char* s = std::getenv("QVNT_UNROLL");
+        int v = s ? std::atoi(s) : 2;
+        return (v == 1 || v == 2 || v == 4) ? v : 2;
+    }();
+    return u;
+}
+
+void add_pos(Deposit& d, unsigned p) {
+    unsigned k = d.n++;
+    while (k > 0 && d.pos[k - 1] > p) {
+        d.pos[k] = d.pos[k - 1];
+        --k;
+    }
+    d.pos[k] = (uint8_t)p;
+}
+void fix_bits(Deposit& d, uint64_t bits, bool one) {
+    for (uint64_t rest = bits; rest; rest &= rest - 1) add_pos(d, ctz(rest));
+    if (one) d.set |= bits;
+}
+
+template <class R>
+Par<R> make_par(const qvnt_gate& g) {
+    Par<R> p{};
+    p.a = g.a;
+    p.b = g.b;
+    p.re = (R)g.re;
+    p.im = (R)g.im;
+    for (int k = 0; k < 32; ++k) p.m[k] = (R)g.m[k];
+    p.dagger = g.dagger ? 1u : 0u;
+    p.i_pow = (~(uint32_t)(popc(g.a) + 1u)) & 3u;  // atomic/y.rs:11
+    return p;
+}
+
+template <class R, int U>
+int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
+    const Par<R> p = make_par<R>(gate);
+    cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
+    const dim3 grid((unsigned)((groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U)));
+    const dim3 block(kBlock);
+    hipStream_t s = r->stream;
+#define QVNT_CASE(KIND, KERNEL)                                            \
+    case KIND:                                                             \
+        hipLaunchKernelGGL((KERNEL<KIND, R, U>), grid, block, 0, s, psi, groups, d, p); \
+        break;
+    if (cls == GClass::Diag) {
+        switch (gate.kind) {
+            QVNT_CASE(QVNT_Z, k_diag)
+            QVNT_CASE(QVNT_S, k_diag)
+            QVNT_CASE(QVNT_T, k_diag)
+            QVNT_CASE(QVNT_RZ, k_diag)
+            QVNT_CASE(QVNT_RZZ, k_diag)
+            QVNT_CASE(QVNT_PHI, k_diag)
+            default: return fail(QVNT_E_ARG, "not a diagonal kind");
+        }
+    } else if (cls == GClass::Pair) {
+        switch (gate.kind) {
+            QVNT_CASE(QVNT_X, k_pair)
+            QVNT_CASE(QVNT_Y, k_pair)
+            QVNT_CASE(QVNT_RX, k_pair)
+            QVNT_CASE(QVNT_RXX, k_pair)
+            QVNT_CASE(QVNT_RY, k_pair)
+            QVNT_CASE(QVNT_RYY, k_pair)
+            QVNT_CASE(QVNT_H1, k_pair)
+            QVNT_CASE(QVNT_U1, k_pair)
+            QVNT_CASE(QVNT_SWAP, k_pair)
+            QVNT_CASE(QVNT_ISWAP, k_pair)
+            QVNT_CASE(QVNT_SQRT_SWAP, k_pair)
+            QVNT_CASE(QVNT_SQRT_ISWAP, k_pair)
+            default: return fail(QVNT_E_ARG, "not a pair kind");
+        }
+    } else {
+        switch (gate.kind) {
+            QVNT_CASE(QVNT_H2, k_quad)
+            QVNT_CASE(QVNT_U2, k_quad)
+            default: return fail(QVNT_E_ARG, "not a quad kind");
+        }
+    }
+#undef QVNT_CASE
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+template <class R>
+int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
+    switch (unroll_setting()) {
+    case 1: return launch_kernel<R, 1>(r, gate, cls, groups, d);
+    case 4: return launch_kernel<R, 4>(r, gate, cls, groups, d);
+    default: return launch_kernel<R, 2>(r, gate, cls, groups, d);
+    }
+}
+
+}  // namespace
+
+int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
+    const uint64_t lmask = r->len - 1u;
+    const uint32_t k = gate.kind;
+    if (k >= QVNT_KIND_COUNT) return fail(QVNT_E_ARG, "unknown gate kind");
+    // A control bit outside the buffer is never set: the predicate
+    // `!idx & ctrl == 0` (dispatch.rs:35) fails everywhere, the gate is the
+    // identity.
+    if (gate.ctrl & ~lmask) return QVNT_OK;
+    if (k == QVNT_ID) return QVNT_OK;
+
+    const bool diag = is_diagonal_kind(k);
+    const bool quad = (k == QVNT_H2 || k == QVNT_U2);
+    const uint64_t touched = touched_bits(gate);
+    // Bit-count rules of the checked constructors (atomic/<gate>.rs is_valid).
+    switch (k) {
+    case QVNT_RX: case QVNT_RY: case QVNT_RZ: case QVNT_H1: case QVNT_U1: case QVNT_PHI:
+        if (popc(gate.a) != 1) return fail(QVNT_E_INVALID_MASK, "Mask should contain 1 bit!");
+        break;
+    case QVNT_RXX: case QVNT_RYY: case QVNT_RZZ:
+    case QVNT_SWAP: case QVNT_ISWAP: case QVNT_SQRT_SWAP: case QVNT_SQRT_ISWAP:
+        if (popc(gate.a) != 2) return fail(QVNT_E_INVALID_MASK, "Mask should contain 2 bit!");
+        break;
+    case QVNT_H2: case QVNT_U2:
+        if (popc(gate.a) != 1 || popc(gate.b) != 1 || gate.a == gate.b)
+            return fail(QVNT_E_INVALID_MASK, "H2/U2 need two distinct single-bit masks");
+        break;
+    default: break;
+    }
+    if (gate.ctrl & touched) return fail(QVNT_E_CTRL_OVERLAP, "control mask overlaps the gate's qubits");
+    // Non-diagonal gates read psi[idx ^ mask]; beyond the buffer the
+    // reference indexes out of bounds (a panic).  Diagonal gates only test
+    // idx & mask, which is zero outside the buffer.
+    if (!diag && (touched & ~lmask)) return fail(QVNT_E_ARG, "gate acts on qubits beyond the register");
+
+    Deposit d{};
+    GClass cls;
+    const uint64_t a_in = gate.a & lmask;
+    if (diag) {
+        cls = GClass::Diag;
+        const bool single_bit = popc(a_in) == 1;
+        const bool zero_fixed = (k == QVNT_Z || k == QVNT_S || k == QVNT_T || k == QVNT_PHI);
+        if (zero_fixed && a_in == 0) return QVNT_OK;               // identity
+        if (zero_fixed && single_bit) fix_bits(d, a_in, true);     // the |0> half is unchanged
+    } else if (quad) {
+        cls = GClass::Quad;
+        fix_bits(d, gate.a | gate.b, false);
+    } else {
+        cls = GClass::Pair;
+        if (gate.a == 0) return QVNT_OK;                           // X0 / Y0: identity
+        const uint64_t hi = 1ull << msb(gate.a);
+        fix_bits(d, hi, false);
+        if (k == QVNT_SWAP || k == QVNT_ISWAP || k == QVNT_SQRT_SWAP || k == QVNT_SQRT_ISWAP)
+            fix_bits(d, gate.a & ~hi, true);                       // odd-parity orbits only
+    }
+    fix_bits(d, gate.ctrl, true);
+    const uint64_t groups = r->len >> d.n;
+
+    const double bytes = 2.0 * (double)r->amp_bytes() * (double)(r->len >> popc(gate.ctrl));
+    const int kc = cls == GClass::Diag ? QVNT_K_DIAG : cls == GClass::Pair ? QVNT_K_PAIR : QVNT_K_QUAD;
+    Timed t(r, kc, bytes);
+    if (r->prec == QVNT_F64) return launch_unrolled<double>(r, gate, cls, groups, d);
+    return launch_unrolled<float>(r, gate, cls, groups, d);
+}
+
+int launch_init(qvnt_reg* r, uint64_t state) {
+    const dim3 grid((unsigned)((r->len + kBlock - 1) / kBlock));
+    Timed t(r, QVNT_K_OTHER, (double)r->amp_bytes() * (double)r->len);
+    if (r->prec == QVNT_F64)
+        hipLaunchKernelGGL(k_init<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), r->len, state);
+    else
+        hipLaunchKernelGGL(k_init<float>, grid, dim3(kBlock), 0, r->stream, static_cast<c32*>(r->psi), r->len, state);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+}  // namespace qvnt

@@ -1,0 +1,243 @@
+// C ABI of the device register: QReg::new / with_state / apply /
+// get_absolute / get_probabilities / measure_mask (src/register/quant.rs),
+// plus state copy-in/out and per-kernel-class timing.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "reg.hpp"
+
+namespace qvnt {
+
+Timed::Timed(qvnt_reg* reg, int kc, double by) : r(reg), kclass(kc), bytes(by) {
+    if (!r->timing) return;
+    for (hipEvent_t* e : {&a, &b}) {
+        if (!r->event_pool.empty()) {
+            *e = r->event_pool.back();
+            r->event_pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            *e = nullptr;
+        }
+    }
+    if (a) (void)hipEventRecord(a, r->stream);
+}
+
+Timed::~Timed() {
+    if (!r->timing || !a || !b) return;
+    (void)hipEventRecord(b, r->stream);
+    r->pending.push_back({kclass, a, b, bytes});
+}
+
+namespace {
+
+int fold_timing(qvnt_reg* r) {
+    if (r->pending.empty()) return QVNT_OK;
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    for (auto& p : r->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            r->stats[p.kclass].launches += 1;
+            r->stats[p.kclass].ms += ms;
+            r->stats[p.kclass].bytes += p.bytes;
+        }
+        r->event_pool.push_back(p.a);
+        r->event_pool.push_back(p.b);
+    }
+    r->pending.clear();
+    return QVNT_OK;
+}
+
+// Only gfx950 code objects are built; refuse any other device up front.
+int check_device(int* dev) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(QVNT_E_NO_DEVICE, "no HIP device visible");
+    QVNT_HIP(hipGetDevice(dev));
+    hipDeviceProp_t prop;
+    QVNT_HIP(hipGetDeviceProperties(&prop, *dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(QVNT_E_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    return QVNT_OK;
+}
+
+int check_reg(const qvnt_reg* r) { return r ? QVNT_OK : fail(QVNT_E_ARG, "null register"); }
+
+}  // namespace
+}  // namespace qvnt
+
+using namespace qvnt;
+
+extern "C" {
+
+int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg** out) {
+    if (!out) return fail(QVNT_E_ARG, "null output pointer");
+    if (prec != QVNT_F64 && prec != QVNT_F32) return fail(QVNT_E_ARG, "unknown precision");
+    if (q_num > 40) return fail(QVNT_E_OOM, "register too large");
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    qvnt_reg* r = new (std::nothrow) qvnt_reg;
+    if (!r) return fail(QVNT_E_OOM, "host allocation failed");
+    r->q_num = q_num;
+    r->ns = q_num < 3 ? 3 : q_num;
+    r->len = 1ull << r->ns;
+    r->prec = prec;
+    r->device = dev;
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete r;
+        return fail(QVNT_E_HIP, "stream creation failed");
+    }
+    if (hipMalloc(&r->psi, r->len * r->amp_bytes()) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(r->stream);
+        delete r;
+        return fail(QVNT_E_OOM, "device memory for the state vector");
+    }
+    const uint64_t q_mask = (1ull << q_num) - 1u;
+    rc = launch_init(r, state & q_mask);
+    if (rc) {
+        qvnt_reg_free(r);
+        return rc;
+    }
+    *out = r;
+    clear_error();
+    return QVNT_OK;
+}
+
+int qvnt_reg_new(uint32_t q_num, qvnt_prec prec, qvnt_reg** out) { return qvnt_reg_with_state(q_num, 0, prec, out); }
+
+int qvnt_reg_free(qvnt_reg* r) {
+    if (!r) return QVNT_OK;
+    (void)hipStreamSynchronize(r->stream);
+    for (auto& p : r->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
+    if (r->psi) (void)hipFree(r->psi);
+    if (r->tree) (void)hipFree(r->tree);
+    if (r->dscratch) (void)hipFree(r->dscratch);
+    if (r->iscratch) (void)hipFree(r->iscratch);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return QVNT_OK;
+}
+
+uint32_t qvnt_reg_num(const qvnt_reg* r) { return r ? r->q_num : 0; }
+uint64_t qvnt_reg_len(const qvnt_reg* r) { return r ? r->len : 0; }
+qvnt_prec qvnt_reg_prec(const qvnt_reg* r) { return r ? r->prec : QVNT_F64; }
+
+int qvnt_reg_apply_gates(qvnt_reg* r, const qvnt_gate* gates, size_t count) {
+    if (int rc = check_reg(r)) return rc;
+    if (count && !gates) return fail(QVNT_E_ARG, "null gate array");
+    for (size_t i = 0; i < count; ++i) {
+        int rc = launch_gate(r, gates[i]);
+        if (rc) return rc;
+    }
+    return QVNT_OK;
+}
+
+int qvnt_reg_apply(qvnt_reg* r, const qvnt_op* op) {
+    if (!op) return fail(QVNT_E_ARG, "null op");
+    return qvnt_reg_apply_gates(r, op->g.data(), op->g.size());
+}
+
+int qvnt_reg_get_absolute(qvnt_reg* r, double* out) {
+    if (int rc = check_reg(r)) return rc;
+    if (!out) return fail(QVNT_E_ARG, "null output");
+    return tree_build(r, r->len, out);
+}
+
+int qvnt_reg_get_probabilities(qvnt_reg* r, double* dst, uint64_t len) {
+    if (int rc = check_reg(r)) return rc;
+    const uint64_t n = 1ull << r->q_num;
+    if (!dst || len < n) return fail(QVNT_E_ARG, "destination must hold 2^n doubles");
+    double total = 0.0;
+    int rc = tree_build(r, r->len, &total);
+    if (rc) return rc;
+    return probabilities(r, 1.0 / total, dst, n);
+}
+
+int qvnt_reg_sample_mask(qvnt_reg* r, uint64_t mask, const double* u, size_t shots, uint64_t* out) {
+    if (int rc = check_reg(r)) return rc;
+    if (shots == 0) return QVNT_OK;
+    if (!u || !out) return fail(QVNT_E_ARG, "null draws or outcomes");
+    if (shots > (1u << 20)) return fail(QVNT_E_ARG, "at most 2^20 shots per call");
+    mask &= (1ull << r->q_num) - 1u;
+    if (mask == 0) {
+        for (size_t k = 0; k < shots; ++k) out[k] = 0;
+        return QVNT_OK;
+    }
+    for (size_t k = 0; k < shots; ++k)
+        if (!(u[k] >= 0.0 && u[k] < 1.0)) return fail(QVNT_E_ARG, "draws must lie in [0, 1)");
+    double total = 0.0;
+    int rc = tree_sample(r, 1ull << r->q_num, u, shots, out, &total);
+    if (rc) return rc;
+    if (!(total > 0.0) || !std::isfinite(total)) return fail(QVNT_E_ZERO_NORM, "measurement of a zero-norm or non-finite state");
+    for (size_t k = 0; k < shots; ++k) out[k] &= mask;
+    return QVNT_OK;
+}
+
+int qvnt_reg_measure_mask(qvnt_reg* r, uint64_t mask, double u, uint32_t flags, uint64_t* out) {
+    if (int rc = check_reg(r)) return rc;
+    if (!out) return fail(QVNT_E_ARG, "null output");
+    mask &= (1ull << r->q_num) - 1u;   // quant.rs:491
+    if (mask == 0) {                   // quant.rs:492-494: no draw, no collapse
+        *out = 0;
+        return QVNT_OK;
+    }
+    if (!(u >= 0.0 && u < 1.0)) return fail(QVNT_E_ARG, "draw must lie in [0, 1)");
+    uint64_t outcome = 0;
+    double total = 0.0;
+    int rc = tree_sample(r, 1ull << r->q_num, &u, 1, &outcome, &total);
+    if (rc) return rc;
+    if (!(total > 0.0) || !std::isfinite(total)) return fail(QVNT_E_ZERO_NORM, "measurement of a zero-norm or non-finite state");
+    rc = collapse(r, outcome, mask, (flags & QVNT_MEASURE_RENORM) != 0);
+    if (rc) return rc;
+    *out = outcome & mask;             // quant.rs:500
+    return QVNT_OK;
+}
+
+int qvnt_reg_measure(qvnt_reg* r, double u, uint32_t flags, uint64_t* out) {
+    if (int rc = check_reg(r)) return rc;
+    return qvnt_reg_measure_mask(r, (1ull << r->q_num) - 1u, u, flags, out);
+}
+
+int qvnt_reg_get_state(qvnt_reg* r, void* dst, size_t bytes) {
+    if (int rc = check_reg(r)) return rc;
+    if (!dst || bytes != r->len * r->amp_bytes()) return fail(QVNT_E_ARG, "state copy needs len * amplitude bytes");
+    QVNT_HIP(hipMemcpyAsync(dst, r->psi, bytes, hipMemcpyDeviceToHost, r->stream));
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    return QVNT_OK;
+}
+
+int qvnt_reg_set_state(qvnt_reg* r, const void* src, size_t bytes) {
+    if (int rc = check_reg(r)) return rc;
+    if (!src || bytes != r->len * r->amp_bytes()) return fail(QVNT_E_ARG, "state copy needs len * amplitude bytes");
+    QVNT_HIP(hipMemcpyAsync(r->psi, src, bytes, hipMemcpyHostToDevice, r->stream));
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    return QVNT_OK;
+}
+
+int qvnt_reg_sync(qvnt_reg* r) {
+    if (int rc = check_reg(r)) return rc;
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    return QVNT_OK;
+}
+
+int qvnt_reg_set_timing(qvnt_reg* r, int enable) {
+    if (int rc = check_reg(r)) return rc;
+    if (int rc = fold_timing(r)) return rc;
+    r->timing = enable != 0;
+    return QVNT_OK;
+}
+
+int qvnt_reg_kernel_stats(qvnt_reg* r, qvnt_kstats* out, int reset) {
+    if (int rc = check_reg(r)) return rc;
+    if (int rc = fold_timing(r)) return rc;
+    if (out) std::memcpy(out, r->stats, sizeof(r->stats));
+    if (reset) std::memset(r->stats, 0, sizeof(r->stats));
+    return QVNT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// The device register (the reference's QReg, src/register/quant.rs:103-108)
+// and the launch helpers shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+#define QVNT_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t qvnt_e_ = (call);                                                         \
+        if (qvnt_e_ != hipSuccess)                                                           \
+            return ::qvnt::fail(QVNT_E_HIP, std::string(#call) + ": " + hipGetErrorString(qvnt_e_)); \
+    } while (0)
+
+struct qvnt_reg {
+    uint32_t q_num = 0;      // qubits of the register
+    uint32_t ns = 0;         // storage qubits: log2(len) = max(q_num, 3)
+    uint64_t len = 0;        // stored amplitudes, max(2^q_num, 8) (quant.rs:15,116)
+    qvnt_prec prec = QVNT_F64;
+    int device = 0;
+    void* psi = nullptr;     // device amplitudes, interleaved complex
+    hipStream_t stream = nullptr;
+
+    // measurement workspace: stored levels of the |psi|^2 pairwise tree
+    double* tree = nullptr;
+    size_t tree_cap = 0;     // doubles
+    double* dscratch = nullptr;      // small device scratch (draws, totals)
+    uint64_t* iscratch = nullptr;    // small device scratch (outcomes)
+    size_t shots_cap = 0;
+
+    // per-kernel-class timing
+    bool timing = false;
+    struct Pending { int kclass; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    qvnt_kstats stats[QVNT_KCLASS_COUNT] = {};
+
+    size_t amp_bytes() const { return prec == QVNT_F64 ? 16 : 8; }
+};
+
+namespace qvnt {
+
+// Brackets one launch with timing events when the register has timing on.
+struct Timed {
+    qvnt_reg* r;
+    int kclass;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(qvnt_reg* reg, int kc, double by);
+    ~Timed();
+};
+
+// plain.hip
+int launch_gate(qvnt_reg* r, const qvnt_gate& g);
+int launch_init(qvnt_reg* r, uint64_t state);
+
+// measure.hip
+int tree_build(qvnt_reg* r, uint64_t count, double* total_out);
+int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out);
+int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm);
+int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n);
+
+}  // namespace qvnt

@@ -1,0 +1,282 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Plain FP64 apply is compared bit for bit (the kernels restate the
+reference's arithmetic with contraction off); the stated contract is
+max |dpsi| <= 1e-12.  Measured outcomes, masks and indices are compared
+exactly for the same seeded draws."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import qvnt_amd
+from qvnt_amd import QReg, circuits, op
+from golden_util import build_ops, load_known_answers, symbol_table
+
+pytestmark = pytest.mark.gpu
+
+KA = load_known_answers()
+TABLE = symbol_table(KA["angle"])
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+class ProductNS:
+    def __getattr__(self, name):
+        return getattr(op, name)
+
+    def raw_h2(self, a, b):
+        g = qvnt_amd.Gate()
+        g.kind, g.act, g.a, g.b = qvnt_amd.KIND["H2"], a | b, a, b
+        return qvnt_amd.from_gates([g])
+
+
+def gpu_matrix(o, size):
+    dim = 1 << size
+    cols = []
+    for col in range(dim):
+        with QReg.with_state(size, col) as r:
+            r.apply(o)
+            cols.append(r.get_state()[:dim])
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("case", KA["matrix_cases"], ids=lambda c: c["ref"].split("/")[-1] + ":" + c.get("name", "Id")[:20])
+def test_matrix_known_answers(case):
+    say("matrix case", case["ref"], case.get("name"))
+    o = build_ops(case["ops"], ProductNS(), TABLE, KA["angle"])
+    if case.get("dgr"):
+        o = o.dgr()
+    got = gpu_matrix(o, case["size"])
+    want = np.array([[TABLE[v] for v in row] for row in case["matrix"]])
+    assert np.array_equal(got, want)
+
+
+def test_golden_state_vector():
+    case = KA["state_cases"][0]
+    o = build_ops(case["ops"], ProductNS(), TABLE, KA["angle"])
+    with QReg.with_state(case["q_num"], case["state"]) as r:
+        r.apply(o)
+        want = np.array([complex(re, im) for re, im in case["psi"]])
+        assert np.array_equal(r.get_state(), want)
+        mask = case["measure_mask"]
+        assert r.measure_mask(mask).get() & ~mask == 0
+
+
+@pytest.mark.parametrize("case", KA["probability_cases"], ids=lambda c: c["ref"])
+def test_probabilities_known_answers(case):
+    o = build_ops(case["ops"], ProductNS(), TABLE, KA["angle"])
+    with QReg.with_state(case["q_num"], case["state"]) as r:
+        r.apply(o)
+        p = r.get_probabilities()
+    want = np.array(case["probabilities"])
+    if case["tol"] == 0.0:
+        assert np.array_equal(p, want)
+    else:
+        assert np.max(np.abs(p - want)) < case["tol"]
+
+
+KINDS = ["x", "y", "z", "s", "t", "rx", "ry", "rz", "rxx", "ryy", "rzz", "phi",
+         "swap", "i_swap", "sqrt_swap", "sqrt_i_swap", "h", "u3", "qft"]
+
+
+def random_circuit(g, n, count):
+    specs = []
+    for _ in range(count):
+        ctor = KINDS[int(g.integers(len(KINDS)))]
+        perm = [int(v) for v in g.permutation(n)]
+        if ctor in ("x", "y", "z", "s", "t"):
+            k = int(g.integers(1, min(n, 3) + 1))
+            args, rest = (sum(1 << q for q in perm[:k]),), perm[k:]
+        elif ctor in ("rx", "ry", "rz", "phi"):
+            args, rest = (float(g.uniform(0, 2 * math.pi)), 1 << perm[0]), perm[1:]
+        elif ctor == "u3":
+            args, rest = tuple(float(v) for v in g.uniform(0, 2 * math.pi, 3)) + (1 << perm[0],), perm[1:]
+        elif ctor in ("h", "qft"):
+            k = int(g.integers(1, min(n, 4) + 1))
+            args, rest = (sum(1 << q for q in perm[:k]),), perm[k:]
+        elif n >= 2:
+            m = (1 << perm[0]) | (1 << perm[1])
+            args, rest = ((float(g.uniform(0, 2 * math.pi)), m) if ctor.startswith("r") else (m,)), perm[2:]
+        else:
+            continue
+        nc = int(g.integers(0, min(len(rest), 2) + 1))
+        specs.append((ctor, args, sum(1 << q for q in rest[:nc]), bool(g.integers(0, 4) == 0)))
+    return specs
+
+
+def build(specs, ns):
+    out = ns.id()
+    for ctor, args, ctrl, dag in specs:
+        o = getattr(ns, ctor)(*args)
+        if dag:
+            o = o.dgr()
+        if ctrl:
+            o = o.c(ctrl)
+        out = out * o
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 9, 11, 12, 13, 16])
+def test_random_circuits_bit_exact(orc, n):
+    """Every gate kind, with controls and daggers, on sizes around the
+    2^3 buffer minimum and the kernel tile sizes."""
+    say("random circuit n =", n)
+    g = np.random.default_rng(1000 + n)
+    specs = random_circuit(g, n, 120)
+    state = int(g.integers(0, 1 << n))
+    ref = orc.Reg(n, state)
+    ref.apply(build(specs, orc.op))
+    with QReg.with_state(n, state) as r:
+        r.apply(build(specs, op))
+        got = r.get_state()
+    want = ref.state()
+    assert np.max(np.abs(got - want)) <= TOL
+    assert np.array_equal(got, want)
+
+
+def test_random_state_input_every_kind(orc):
+    """Arbitrary (non-basis) input states through set_state."""
+    g = np.random.default_rng(77)
+    n = 10
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    specs = random_circuit(g, n, 200)
+    ref = orc.Reg(n)
+    ref.set_state(psi)
+    ref.apply(build(specs, orc.op))
+    with QReg(n) as r:
+        r.set_state(psi)
+        r.apply(build(specs, op))
+        assert np.array_equal(r.get_state(), ref.state())
+
+
+def test_plain_20q_workload(orc):
+    """The 20-qubit plain workload: qft, h, 200 random gates, measure_mask(0xFF)."""
+    say("plain_20q")
+    state, specs, mask = circuits.plain_20q()
+    ref = orc.Reg(20, state, threads=8)
+    ref.apply(circuits.build(specs, orc.op))
+    with QReg.with_state(20, state) as r:
+        r.apply(circuits.build(specs, op))
+        got = r.get_state()
+        want = ref.state()
+        assert np.max(np.abs(got - want)) <= TOL
+        assert np.array_equal(got, want)
+        u = qvnt_amd.uniform(circuits.SEED, 0, 0)
+        c = r.measure_mask(mask, u=u)
+        rc = ref.measure_mask(mask, u)
+        assert c.get() == rc and rc & ~mask == 0
+        assert np.array_equal(r.get_state(), ref.state())
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 6, 10, 11, 12, 15, 22, 23])
+def test_measure_matches_oracle(orc, n):
+    """Same state, same draw -> same outcome, same collapsed state (both
+    renormalised and not), same totals."""
+    say("measure n =", n)
+    g = np.random.default_rng(300 + n)
+    length = max(1 << n, 8)
+    psi = np.zeros(length, complex)
+    psi[: 1 << n] = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi[: 1 << n][g.random(1 << n) < 0.3] = 0
+    if not psi.any():
+        psi[0] = 1
+    masks = [(1 << n) - 1, 0b101 & ((1 << n) - 1), 1 << max(n - 1, 0), 0]
+    for k, mask in enumerate(masks):
+        for renorm in (False, True):
+            u = qvnt_amd.uniform(n, k, int(renorm))
+            ref = orc.Reg(n)
+            ref.set_state(psi)
+            want_total = ref.get_absolute()
+            with QReg(n) as r:
+                r.set_state(psi)
+                tot = r.get_absolute()
+                assert tot == orc.tree_total(psi)
+                assert abs(tot - want_total) <= 1e-12 * want_total
+                got = r.measure_mask(mask, u=u, renorm=renorm).get()
+                assert got == ref.measure_mask(mask, u, renorm=renorm)
+                assert np.array_equal(r.get_state(), ref.state())
+
+
+def test_probabilities_and_shots(orc):
+    g = np.random.default_rng(5)
+    n = 14
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    ref = orc.Reg(n)
+    ref.set_state(psi)
+    want = ref.get_probabilities()
+    us = np.array([qvnt_amd.uniform(9, 1, k) for k in range(1024)])
+    with QReg(n) as r:
+        r.set_state(psi)
+        p = r.get_probabilities()
+        assert np.max(np.abs(p - want)) <= 1e-12 * np.max(want)
+        mask = 0b1010_0110_0001
+        shots = r.sample_mask(mask, us)
+        assert np.array_equal(r.get_state(), psi.astype(complex))  # non-destructive
+    for k in range(0, 1024, 37):
+        assert shots[k] == orc.tree_sample(psi, us[k]) & mask
+
+
+def test_zero_state_and_bad_gates():
+    with QReg(5) as r:
+        r.set_state(np.zeros(32, complex))
+        with pytest.raises(qvnt_amd.QvntError) as e:
+            r.measure_mask(0b11, u=0.5)
+        assert e.value.code == -5
+    with QReg(4) as r:
+        with pytest.raises(qvnt_amd.QvntError):
+            r.apply(op.x(1 << 6))       # beyond the register: the reference panics
+        r.apply(op.rz(0.3, 1 << 9))     # diagonal beyond: a global phase, as in the reference
+        r.apply(op.x(1).c(1 << 8))      # control beyond: never satisfied, identity
+        s = r.get_state()
+        assert abs(s[0] - complex(math.cos(0.15), -math.sin(0.15))) < 1e-15
+
+
+def test_fp32_mode_error_bound(orc):
+    """FP32 amplitudes: error against the FP64 oracle on the 20-qubit workload."""
+    state, specs, mask = circuits.plain_20q()
+    ref = orc.Reg(20, state, threads=8)
+    ref.apply(circuits.build(specs, orc.op))
+    want = ref.state()
+    with QReg.with_state(20, state, prec="f32") as r:
+        r.apply(circuits.build(specs, op))
+        got = r.get_state().astype(np.complex128)
+        assert abs(r.get_absolute() - 1.0) < 1e-5
+    err = np.max(np.abs(got - want))
+    infid = 1.0 - abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    say("fp32 max err", err, "infidelity", infid)
+    assert err < 1e-5 and infid < 1e-5
+
+
+@pytest.mark.parametrize("n", [26, 30])
+def test_large_register_round_trip(n):
+    """Size-independent properties at full size: a brickwork layer and its
+    exact inverse (negated angles) restore the basis state; norms hold."""
+    say("round trip n =", n)
+    fwd, inv = [], []
+    for layer in range(2):
+        specs = circuits.brickwork_layer(n, layer)
+        fwd += specs
+    for ctor, args, ctrl in reversed(fwd):
+        if ctor == "u3":
+            th, ph, la, m = args
+            inv += [("rz", (-ph, m), ctrl), ("ry", (-th, m), ctrl), ("rz", (-la, m), ctrl)]
+        elif ctor == "rzz":
+            inv.append(("rzz", (-args[0], args[1]), ctrl))
+        else:
+            inv.append((ctor, args, ctrl))
+    start = 0x2AAAAAAA & ((1 << n) - 1)
+    with QReg.with_state(n, start) as r:
+        r.apply(circuits.build(fwd, op))
+        assert abs(r.get_absolute() - 1.0) < 1e-12
+        r.apply(circuits.build(inv, op))
+        assert abs(r.get_absolute() - 1.0) < 1e-12
+        draws = [qvnt_amd.uniform(4, n, k) for k in range(16)]
+        outs = r.sample_mask((1 << n) - 1, draws)
+        assert all(int(o) == start for o in outs)
