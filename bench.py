@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: gates/s and % of HBM roofline of the plain FP64 apply on the
-30-qubit brickwork random circuit (16 GiB state), one MI355X.
+30-qubit brickwork random circuit (16 GiB state), one MI355X, with the
+fused form on the same layers beside it (key "fused").
 
 A step is one brickwork layer: u3 on every qubit (3 SingleOps each), rzz on
 alternating nearest-neighbour pairs, one controlled x -- 105 or 106
@@ -58,6 +59,46 @@ def cpu_baseline(n, budget_s):
                       "out-of-place gather + per-apply copy), %.1f s" % (done, n, elapsed)}
 
 
+def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value):
+    """The same layers through qvnt_reg_apply_fused, one call per layer,
+    then the whole timed circuit as one call."""
+    out = {}
+    for label, groups in (("per_layer", [[o] for o in ops[warmup:]]), ("whole_circuit", [ops[warmup:]])):
+        reg = QReg(n)
+        for o in ops[:warmup]:
+            reg.apply_fused(o)
+        merged = []
+        for grp in groups:
+            m = grp[0].clone()
+            for o in grp[1:]:
+                m *= o
+            merged.append(m)
+        reg.sync()
+        reg.set_timing(True)
+        reg.kernel_stats(reset=True)
+        passes = 0
+        t0 = time.perf_counter()
+        for m in merged:
+            passes += reg.apply_fused(m)
+        reg.sync()
+        wall = time.perf_counter() - t0
+        st = reg.kernel_stats(reset=True)["fused"]
+        norm = reg.get_absolute()
+        reg.close()
+        v = timed_gates / wall
+        ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] else 0.0
+        log("fused (%s): %d passes for %d SingleOps in %.3f s -> %.1f gates/s (x%.1f plain); k_fused %.1f GB/s; |psi|^2-1 = %.1e"
+            % (label, passes, timed_gates, wall, v, v / plain_value, ach, norm - 1.0))
+        out[label] = {"value": round(v, 3), "passes": passes, "seconds": round(wall, 4),
+                      "speedup_vs_plain": round(v / plain_value, 2),
+                      "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_fused",
+                                   "launches": st["launches"],
+                                   "avg_ms": round(st["ms"] / st["launches"], 4) if st["launches"] else None,
+                                   "bytes_per_launch": st["bytes"] / st["launches"] if st["launches"] else None}}
+    return out
+
+
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -76,6 +117,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fused", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -126,6 +168,10 @@ def main():
     all_ms = sum(s["ms"] for s in stats.values())
     all_bytes = sum(s["bytes"] for s in stats.values())
 
+    fused = None
+    if not args.no_fused:
+        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, value)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, host cores) ...")
@@ -142,6 +188,7 @@ def main():
                    "form": "plain (one in-place kernel per SingleOp)"},
         "roofline": roofline,
         "all_kernels_gbs": round(all_bytes / all_ms / 1e6, 1) if all_ms else None,
+        "fused": fused,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -180,6 +180,16 @@ int qvnt_reg_apply(qvnt_reg* reg, const qvnt_op* op);
 /* Same for a raw gate array (borrowed for the call). */
 int qvnt_reg_apply_gates(qvnt_reg* reg, const qvnt_gate* gates, size_t count);
 
+/* The fused form of QReg::apply (no reference counterpart: the reference
+ * runs one gather pass per SingleOp).  The queue is scheduled into passes;
+ * each pass reads and writes the state once and applies every gate it
+ * absorbed while tiles of 2^t amplitudes (t = max_qubits, at most 12; 0 =
+ * 12) sit in LDS and registers.  Gates are reordered only past gates on
+ * disjoint qubits.  Same result as qvnt_reg_apply to rounding (FMA and
+ * merged 2x2 products: about 1e-15 per gate).  *passes (may be NULL) gets
+ * the number of passes over the state. */
+int qvnt_reg_apply_fused(qvnt_reg* reg, const qvnt_op* op, uint32_t max_qubits, uint32_t* passes);
+
 /* QReg::get_absolute (quant.rs:458-466): sum of |psi_i|^2 over the buffer. */
 int qvnt_reg_get_absolute(qvnt_reg* reg, double* out);
 /* QReg::get_probabilities (quant.rs:434-454): p_i = |psi_i|^2 / total for

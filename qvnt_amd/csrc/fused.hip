@@ -1,0 +1,249 @@
+// The fused form of QReg::apply: many SingleOps per pass over the state.
+//
+// The host compiler (fusion.cpp) turns the queue into passes; this kernel
+// runs one pass.  One block per tile of 2^t amplitudes (the tile's t qubits
+// are the pass's qubit set).  The tile is read once from HBM, every phase
+// moves its 16 amplitudes per thread LDS -> registers -> LDS (skipped when
+// the phase's register qubits are the tile's top four, the layout of the
+// HBM access itself), and the tile is written back once.  HBM traffic is
+// one read and one write of the state per pass, whatever the number of
+// gates in it.  Results agree with the plain form to rounding (FMA
+// contraction is on in this file; 2x2 products are merged on the host).
+#pragma clang fp contract(fast)
+
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "fusion.hpp"
+#include "reg.hpp"
+
+namespace qvnt {
+namespace {
+
+using namespace fusion;
+
+__device__ __forceinline__ uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
+
+template <class R>
+__device__ __forceinline__ cplx<R> fma_c(cplx<R> acc, cplx<R> a, cplx<R> x) {
+    acc.re += a.re * x.re - a.im * x.im;
+    acc.im += a.re * x.im + a.im * x.re;
+    return acc;
+}
+template <class R>
+__device__ __forceinline__ cplx<R> mul_c(cplx<R> a, cplx<R> x) {
+    return {a.re * x.re - a.im * x.im, a.re * x.im + a.im * x.re};
+}
+template <class R>
+__device__ __forceinline__ cplx<R> ld_coef(const double* c, int k) {
+    return {(R)c[2 * k], (R)c[2 * k + 1]};
+}
+
+template <int J, class R>
+__device__ __forceinline__ void apply_m2(cplx<R>* v, const double* c, uint32_t pc) {
+    const cplx<R> u00 = ld_coef<R>(c, 0), u01 = ld_coef<R>(c, 1), u10 = ld_coef<R>(c, 2), u11 = ld_coef<R>(c, 3);
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & (1 << J)) continue;
+        if (!((pc >> i) & 1u)) continue;
+        const cplx<R> x0 = v[i], x1 = v[i | (1 << J)];
+        v[i] = fma_c(mul_c(u00, x0), u01, x1);
+        v[i | (1 << J)] = fma_c(mul_c(u10, x0), u11, x1);
+    }
+}
+
+template <int J0, int J1, class R>
+__device__ __forceinline__ void apply_m4(cplx<R>* v, const double* c, uint32_t pc) {
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & ((1 << J0) | (1 << J1))) continue;
+        if (!((pc >> i) & 1u)) continue;
+        const int idx[4] = {i, i | (1 << J0), i | (1 << J1), i | (1 << J0) | (1 << J1)};
+        cplx<R> x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cplx<R> acc = mul_c(ld_coef<R>(c, 4 * r), x[0]);
+            acc = fma_c(acc, ld_coef<R>(c, 4 * r + 1), x[1]);
+            acc = fma_c(acc, ld_coef<R>(c, 4 * r + 2), x[2]);
+            acc = fma_c(acc, ld_coef<R>(c, 4 * r + 3), x[3]);
+            v[idx[r]] = acc;
+        }
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void apply_d(cplx<R>* v, const double* c, uint32_t pc, uint32_t pa, uint32_t pb) {
+    const cplx<R> t0 = ld_coef<R>(c, 0), t1 = ld_coef<R>(c, 1), t2 = ld_coef<R>(c, 2), t3 = ld_coef<R>(c, 3);
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (!((pc >> i) & 1u)) continue;
+        const bool a = (pa >> i) & 1u, b = (pb >> i) & 1u;
+        const cplx<R> e = b ? (a ? t3 : t2) : (a ? t1 : t0);
+        v[i] = mul_c(e, v[i]);
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FOp* __restrict__ ops,
+                                          const double* __restrict__ coef, uint32_t tb, uint64_t tile) {
+    for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
+        const FOp& op = ops[o];
+        // per-thread control: lane bits of this thread's base, bits of the tile
+        if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
+        const double* c = coef + op.coef;
+        if (op.type == OP_D) {
+            const uint32_t pa = op.a_is_reg ? op.pa : (((tb & op.la) | (tile & op.ga)) ? 0xFFFFu : 0u);
+            const uint32_t pb = op.b_is_reg ? op.pb : (((tb & op.lb) | (tile & op.gb)) ? 0xFFFFu : 0u);
+            apply_d(v, c, op.pc, pa, pb);
+        } else if (op.type == OP_M2) {
+            switch (op.j0) {
+            case 0: apply_m2<0>(v, c, op.pc); break;
+            case 1: apply_m2<1>(v, c, op.pc); break;
+            case 2: apply_m2<2>(v, c, op.pc); break;
+            default: apply_m2<3>(v, c, op.pc); break;
+            }
+        } else {
+            switch (op.j0 * 4 + op.j1) {
+            case 1: apply_m4<0, 1>(v, c, op.pc); break;
+            case 2: apply_m4<0, 2>(v, c, op.pc); break;
+            case 3: apply_m4<0, 3>(v, c, op.pc); break;
+            case 6: apply_m4<1, 2>(v, c, op.pc); break;
+            case 7: apply_m4<1, 3>(v, c, op.pc); break;
+            default: apply_m4<2, 3>(v, c, op.pc); break;
+            }
+        }
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
+                                               const FOp* __restrict__ ops, const double* __restrict__ coef) {
+    extern __shared__ unsigned char smem[];
+    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+    const uint32_t tid = threadIdx.x, nth = pass.nthreads;
+
+    // global index of local 0: insert zeros at the tile's bit positions
+    uint64_t tile = blockIdx.x;
+    for (uint32_t k = 0; k < pass.t; ++k) {
+        const uint64_t lo = tile & ((1ull << pass.tpos[k]) - 1u);
+        tile = ((tile ^ lo) << 1) | lo;
+    }
+    // global offset of this thread's local bits 0..t-5
+    uint64_t lo_off = 0;
+    for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
+
+    cplx<R> v[kRegs];
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) v[j] = psi[tile | lo_off | pass.hoff[j]];
+
+    const uint32_t p0 = pass.phase_begin, p1 = pass.phase_end;
+    for (uint32_t p = p0; p < p1; ++p) {
+        const FPhase& ph = phases[p];
+        // local base of this thread in the phase: zeros at the register bits
+        uint32_t tb = tid;
+#pragma unroll
+        for (int k = 0; k < kRegBits; ++k) {
+            const uint32_t lo = tb & ((1u << ph.rpos[k]) - 1u);
+            tb = ((tb ^ lo) << 1) | lo;
+        }
+        const bool direct_in = (p == p0) && pass.direct_first;
+        const bool direct_out = (p + 1 == p1) && pass.direct_last;
+        if (!direct_in) {
+            if (p == p0) {
+#pragma unroll
+                for (int j = 0; j < kRegs; ++j) lds[swz(tid + j * nth)] = v[j];
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < kRegs; ++i) v[i] = lds[swz(tb | ph.roff[i])];
+        }
+        run_phase(v, ph, ops, coef, tb, tile);
+        if (!direct_out) {
+#pragma unroll
+            for (int i = 0; i < kRegs; ++i) lds[swz(tb | ph.roff[i])] = v[i];
+            __syncthreads();
+            if (p + 1 == p1) {
+#pragma unroll
+                for (int j = 0; j < kRegs; ++j) v[j] = lds[swz(tid + j * nth)];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) psi[tile | lo_off | pass.hoff[j]] = v[j];
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Uploads the program (stream-ordered) and launches one kernel per pass.
+int run_program(qvnt_reg* r, const Program& prog) {
+    static thread_local std::vector<unsigned char> host;
+    const size_t ph_off = 0;
+    const size_t op_off = align256(ph_off + prog.phases.size() * sizeof(FPhase));
+    const size_t cf_off = align256(op_off + prog.ops.size() * sizeof(FOp));
+    const size_t total = cf_off + prog.coef.size() * sizeof(double) + 256;
+    if (total > r->prog_cap) {
+        QVNT_HIP(hipStreamSynchronize(r->stream));
+        if (r->prog) QVNT_HIP(hipFree(r->prog));
+        r->prog = nullptr;
+        r->prog_cap = 0;
+        if (hipMalloc(&r->prog, total) != hipSuccess) return fail(QVNT_E_OOM, "device memory for the fused program");
+        r->prog_cap = total;
+    }
+    host.assign(total, 0);
+    if (!prog.phases.empty()) std::memcpy(host.data() + ph_off, prog.phases.data(), prog.phases.size() * sizeof(FPhase));
+    if (!prog.ops.empty()) std::memcpy(host.data() + op_off, prog.ops.data(), prog.ops.size() * sizeof(FOp));
+    if (!prog.coef.empty()) std::memcpy(host.data() + cf_off, prog.coef.data(), prog.coef.size() * sizeof(double));
+    QVNT_HIP(hipMemcpyAsync(r->prog, host.data(), total, hipMemcpyHostToDevice, r->stream));
+    unsigned char* d = static_cast<unsigned char*>(r->prog);
+    const FPhase* dph = reinterpret_cast<const FPhase*>(d + ph_off);
+    const FOp* dop = reinterpret_cast<const FOp*>(d + op_off);
+    const double* dcf = reinterpret_cast<const double*>(d + cf_off);
+    for (const FPass& fp : prog.passes) {
+        const uint64_t tiles = r->len >> fp.t;
+        const size_t lds = ((size_t)1 << fp.t) * r->amp_bytes();
+        Timed t(r, QVNT_K_FUSED, 2.0 * (double)r->amp_bytes() * (double)r->len);
+        if (r->prec == QVNT_F64)
+            hipLaunchKernelGGL(k_fused<double>, dim3((unsigned)tiles), dim3(fp.nthreads), lds, r->stream,
+                               static_cast<c64*>(r->psi), fp, dph, dop, dcf);
+        else
+            hipLaunchKernelGGL(k_fused<float>, dim3((unsigned)tiles), dim3(fp.nthreads), lds, r->stream,
+                               static_cast<c32*>(r->psi), fp, dph, dop, dcf);
+        QVNT_HIP(hipGetLastError());
+    }
+    return QVNT_OK;
+}
+
+}  // namespace
+
+int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes) {
+    if (passes) *passes = 0;
+    if (r->ns < (unsigned)kMinFused) {
+        for (size_t i = 0; i < n; ++i)
+            if (int rc = launch_gate(r, g[i])) return rc;
+        if (passes) *passes = (uint32_t)n;
+        return QVNT_OK;
+    }
+    unsigned tq = max_qubits ? max_qubits : (unsigned)kMaxTile;
+    if (tq > (unsigned)kMaxTile) tq = kMaxTile;
+    if (tq < (unsigned)kMinFused) tq = kMinFused;
+    if (tq > r->ns) tq = r->ns;
+    std::vector<Step> steps;
+    plan(g, n, r->ns, tq, steps);
+    uint32_t count = 0;
+    for (const Step& st : steps) {
+        if (st.plain) {
+            if (int rc = launch_gate(r, st.gate)) return rc;
+            ++count;
+        } else {
+            if (int rc = run_program(r, st.prog)) return rc;
+            count += (uint32_t)st.prog.passes.size();
+        }
+    }
+    if (passes) *passes = count;
+    return QVNT_OK;
+}
+
+}  // namespace qvnt

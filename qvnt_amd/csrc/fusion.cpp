@@ -1,0 +1,426 @@
+// Host-side compiler of the fused form: lowering, one-qubit merging, pass
+// and phase scheduling, encoding (declarations and overview in fusion.hpp).
+#include "fusion.hpp"
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+
+namespace qvnt {
+namespace fusion {
+namespace {
+
+using cd = std::complex<double>;
+
+struct Prim {
+    uint8_t type;          // OP_M2 / OP_M4 / OP_D
+    int q0 = -1, q1 = -1;  // M2: q0; M4: q0 (matrix bit 0), q1 (matrix bit 1); D: table bits
+    uint64_t ctrl = 0;
+    cd m[16];              // M2 2x2, M4 4x4 (row-major), D table[4]
+    uint64_t targets() const {
+        if (type == OP_D) return 0;
+        return (1ull << q0) | (q1 >= 0 ? (1ull << q1) : 0);
+    }
+    uint64_t bits() const {
+        uint64_t b = ctrl | (1ull << q0);
+        if (q1 >= 0) b |= 1ull << q1;
+        return b;
+    }
+};
+
+constexpr double kR = 0.70710678118654752440;
+
+Prim m2(int q, cd a, cd b, cd c, cd d, uint64_t ctrl) {
+    Prim p;
+    p.type = OP_M2;
+    p.q0 = q;
+    p.ctrl = ctrl;
+    p.m[0] = a; p.m[1] = b; p.m[2] = c; p.m[3] = d;
+    return p;
+}
+Prim diag1(int q, cd e0, cd e1, uint64_t ctrl) {
+    Prim p;
+    p.type = OP_D;
+    p.q0 = q;
+    p.ctrl = ctrl;
+    p.m[0] = e0; p.m[1] = e1; p.m[2] = e0; p.m[3] = e1;
+    return p;
+}
+Prim diag2(int q0, int q1, cd t0, cd t1, cd t2, cd t3, uint64_t ctrl) {
+    Prim p;
+    p.type = OP_D;
+    p.q0 = q0;
+    p.q1 = q1;
+    p.ctrl = ctrl;
+    p.m[0] = t0; p.m[1] = t1; p.m[2] = t2; p.m[3] = t3;
+    return p;
+}
+Prim m4(int q0, int q1, const cd* u, uint64_t ctrl) {
+    Prim p;
+    p.type = OP_M4;
+    p.q0 = q0;
+    p.q1 = q1;
+    p.ctrl = ctrl;
+    for (int k = 0; k < 16; ++k) p.m[k] = u[k];
+    return p;
+}
+
+// Matrices of every reference atomic op from its stored parameters: the
+// same linear map as the gather formulas of atomic/<gate>.rs (the plain
+// kernels), in matrix form.  Returns false if the gate cannot be fused.
+bool lower(const qvnt_gate& g, uint64_t lmask, std::vector<Prim>& out) {
+    const uint64_t ctrl = g.ctrl;
+    const cd c(g.re, 0.0), s(g.im, 0.0), I(0.0, 1.0);
+    const cd e(g.re, g.im);
+    auto bitlist = [](uint64_t m) {
+        std::vector<int> b;
+        for (; m; m &= m - 1) b.push_back((int)ctz(m));
+        return b;
+    };
+    switch (g.kind) {
+    case QVNT_X:
+        for (int q : bitlist(g.a)) out.push_back(m2(q, 0.0, 1.0, 1.0, 0.0, ctrl));
+        return true;
+    case QVNT_Y:  // Y^{(x)k}: one Y per bit
+        for (int q : bitlist(g.a)) out.push_back(m2(q, 0.0, -I, I, 0.0, ctrl));
+        return true;
+    case QVNT_Z:
+        for (int q : bitlist(g.a & lmask)) out.push_back(diag1(q, 1.0, -1.0, ctrl));
+        return true;
+    case QVNT_S:
+        for (int q : bitlist(g.a & lmask)) out.push_back(diag1(q, 1.0, g.dagger ? -I : I, ctrl));
+        return true;
+    case QVNT_T: {
+        const cd t(kR, g.dagger ? -kR : kR);
+        for (int q : bitlist(g.a & lmask)) out.push_back(diag1(q, 1.0, t, ctrl));
+        return true;
+    }
+    case QVNT_RX:
+        out.push_back(m2((int)ctz(g.a), c, -I * s, -I * s, c, ctrl));
+        return true;
+    case QVNT_RY:
+        out.push_back(m2((int)ctz(g.a), c, -s, s, c, ctrl));
+        return true;
+    case QVNT_RZ:
+        if ((g.a & lmask) == 0) {  // qubit beyond the buffer: reads as 0, a global phase
+            out.push_back(diag1(0, std::conj(e), std::conj(e), ctrl));
+            return true;
+        }
+        out.push_back(diag1((int)ctz(g.a), std::conj(e), e, ctrl));
+        return true;
+    case QVNT_PHI:
+        if ((g.a & lmask) == 0) return true;
+        out.push_back(diag1((int)ctz(g.a), 1.0, e, ctrl));
+        return true;
+    case QVNT_H1:
+        out.push_back(m2((int)ctz(g.a), kR, kR, kR, -kR, ctrl));
+        return true;
+    case QVNT_U1:
+        out.push_back(m2((int)ctz(g.a), cd(g.m[0], g.m[1]), cd(g.m[2], g.m[3]), cd(g.m[4], g.m[5]), cd(g.m[6], g.m[7]), ctrl));
+        return true;
+    case QVNT_RZZ: {
+        const std::vector<int> b = bitlist(g.a);
+        const std::vector<int> in = bitlist(g.a & lmask);
+        if (in.size() == 2) {
+            out.push_back(diag2(b[0], b[1], std::conj(e), e, e, std::conj(e), ctrl));
+        } else if (in.size() == 1) {
+            out.push_back(diag1(in[0], std::conj(e), e, ctrl));
+        } else {
+            out.push_back(diag1(0, std::conj(e), std::conj(e), ctrl));
+        }
+        return true;
+    }
+    default: break;
+    }
+    // two-qubit matrices, index bit 0 = lower qubit
+    const std::vector<int> b = bitlist(g.kind == QVNT_H2 || g.kind == QVNT_U2 ? (g.a | g.b) : g.a);
+    cd u[16];
+    for (int k = 0; k < 16; ++k) u[k] = 0.0;
+    auto set = [&](int r, int col, cd v) { u[4 * r + col] = v; };
+    switch (g.kind) {
+    case QVNT_RXX:
+        set(0, 0, c); set(1, 1, c); set(2, 2, c); set(3, 3, c);
+        set(0, 3, -I * s); set(1, 2, -I * s); set(2, 1, -I * s); set(3, 0, -I * s);
+        break;
+    case QVNT_RYY:
+        set(0, 0, c); set(1, 1, c); set(2, 2, c); set(3, 3, c);
+        set(0, 3, I * s); set(1, 2, -I * s); set(2, 1, -I * s); set(3, 0, I * s);
+        break;
+    case QVNT_SWAP:
+        set(0, 0, 1.0); set(1, 2, 1.0); set(2, 1, 1.0); set(3, 3, 1.0);
+        break;
+    case QVNT_ISWAP: {
+        const cd w = g.dagger ? -I : I;
+        set(0, 0, 1.0); set(1, 2, w); set(2, 1, w); set(3, 3, 1.0);
+        break;
+    }
+    case QVNT_SQRT_SWAP: {
+        const cd q(0.5, g.dagger ? -0.5 : 0.5);
+        set(0, 0, 1.0); set(1, 1, q); set(1, 2, std::conj(q)); set(2, 1, std::conj(q)); set(2, 2, q); set(3, 3, 1.0);
+        break;
+    }
+    case QVNT_SQRT_ISWAP: {
+        const cd w(0.0, g.dagger ? -kR : kR);
+        set(0, 0, 1.0); set(1, 1, kR); set(1, 2, w); set(2, 1, w); set(2, 2, kR); set(3, 3, 1.0);
+        break;
+    }
+    case QVNT_H2:
+        for (int r = 0; r < 4; ++r)
+            for (int col = 0; col < 4; ++col) set(r, col, (__builtin_popcount(r & col) & 1) ? -0.5 : 0.5);
+        break;
+    case QVNT_U2: {
+        // reference layout: index bit 0 = a, bit 1 = b (atomic/u2.rs:22-49)
+        if (g.ctrl & g.b) return false;
+        for (int k = 0; k < 16; ++k) u[k] = cd(g.m[2 * k], g.m[2 * k + 1]);
+        out.push_back(m4((int)ctz(g.a), (int)ctz(g.b), u, ctrl));
+        return true;
+    }
+    default:
+        return false;
+    }
+    out.push_back(m4(b[0], b[1], u, ctrl));
+    return true;
+}
+
+void mat2_mul(const cd* a, const cd* b, cd* out) {  // out = a * b
+    cd r[4] = {a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3], a[2] * b[0] + a[3] * b[2], a[2] * b[1] + a[3] * b[3]};
+    for (int k = 0; k < 4; ++k) out[k] = r[k];
+}
+
+// Multiplies runs of uncontrolled one-qubit primitives per qubit.
+std::vector<Prim> merge_single(const std::vector<Prim>& in) {
+    std::vector<Prim> out;
+    struct Pend { bool on = false; cd m[4]; };
+    std::vector<Pend> pend(64);
+    auto flush = [&](int q) {
+        if (!pend[q].on) return;
+        const cd* m = pend[q].m;
+        if (m[1] == 0.0 && m[2] == 0.0) out.push_back(diag1(q, m[0], m[3], 0));
+        else out.push_back(m2(q, m[0], m[1], m[2], m[3], 0));
+        pend[q].on = false;
+    };
+    for (const Prim& p : in) {
+        const bool single = p.ctrl == 0 && p.q1 < 0 && (p.type == OP_M2 || p.type == OP_D);
+        if (single) {
+            cd u[4];
+            if (p.type == OP_M2) { for (int k = 0; k < 4; ++k) u[k] = p.m[k]; }
+            else { u[0] = p.m[0]; u[1] = 0.0; u[2] = 0.0; u[3] = p.m[1]; }
+            Pend& pq = pend[p.q0];
+            if (pq.on) mat2_mul(u, pq.m, pq.m);
+            else { for (int k = 0; k < 4; ++k) pq.m[k] = u[k]; pq.on = true; }
+            continue;
+        }
+        for (uint64_t b = p.bits(); b; b &= b - 1) flush((int)ctz(b));
+        out.push_back(p);
+    }
+    for (int q = 0; q < 64; ++q) flush(q);
+    return out;
+}
+
+struct PassPlan {
+    uint64_t T = 0;
+    std::vector<Prim> prims;
+};
+
+// Greedy pass formation (see the file comment).
+std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsigned tq) {
+    std::vector<PassPlan> passes;
+    const uint64_t forced = (ns >= 3) ? 0x7u : ((1ull << ns) - 1u);
+    while (!rest.empty()) {
+        PassPlan pp;
+        pp.T = forced;
+        uint64_t blocked = 0;
+        std::vector<Prim> leave;
+        for (const Prim& p : rest) {
+            const uint64_t b = p.bits();
+            if (b & blocked) { blocked |= b; leave.push_back(p); continue; }
+            const uint64_t nt = pp.T | p.targets();
+            if (popc(nt) <= tq) { pp.T = nt; pp.prims.push_back(p); }
+            else { blocked |= b; leave.push_back(p); }
+        }
+        // pad T with the lowest free qubits up to tq (larger, fewer tiles)
+        for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
+        passes.push_back(std::move(pp));
+        rest.swap(leave);
+    }
+    return passes;
+}
+
+
+int local_of(const uint8_t* tpos, unsigned t, int q) {
+    for (unsigned k = 0; k < t; ++k)
+        if (tpos[k] == q) return (int)k;
+    return -1;
+}
+
+uint32_t add_coef(Program& prog, const cd* m, int n) {
+    const uint32_t off = (uint32_t)prog.coef.size();
+    for (int k = 0; k < n; ++k) {
+        prog.coef.push_back(m[k].real());
+        prog.coef.push_back(m[k].imag());
+    }
+    return off;
+}
+
+void compile_pass(Program& prog, const PassPlan& pp) {
+    FPass fp{};
+    fp.t = popc(pp.T);
+    fp.nthreads = 1u << (fp.t - kRegBits);
+    {
+        unsigned k = 0;
+        for (uint64_t m = pp.T; m; m &= m - 1) fp.tpos[k++] = (uint8_t)ctz(m);
+    }
+    for (int j = 0; j < kRegs; ++j) {
+        uint64_t off = 0;
+        for (int k = 0; k < kRegBits; ++k)
+            if (j >> k & 1) off |= 1ull << fp.tpos[fp.t - kRegBits + k];
+        fp.hoff[j] = off;
+    }
+    const uint32_t top_regs = ((1u << kRegBits) - 1u) << (fp.t - kRegBits);  // local bits t-4..t-1
+
+    // phases: greedy register sets of four local qubits
+    std::vector<Prim> rest = pp.prims;
+    fp.phase_begin = (uint32_t)prog.phases.size();
+    while (!rest.empty()) {
+        uint32_t Rl = 0;  // local register bits
+        uint64_t blocked = 0;
+        std::vector<Prim> take, leave;
+        for (const Prim& p : rest) {
+            const uint64_t b = p.bits();
+            if (b & blocked) { blocked |= b; leave.push_back(p); continue; }
+            if (p.type == OP_D) { take.push_back(p); continue; }
+            uint32_t need = 0;
+            for (uint64_t m = p.targets(); m; m &= m - 1) need |= 1u << local_of(fp.tpos, fp.t, (int)ctz(m));
+            if (popc(Rl | need) <= kRegBits) { Rl |= need; take.push_back(p); }
+            else { blocked |= b; leave.push_back(p); }
+        }
+        // complete the register set, preferring the top local bits (direct HBM access)
+        for (int k = (int)fp.t - 1; k >= 0 && popc(Rl) < kRegBits; --k)
+            if (top_regs >> k & 1) Rl |= 1u << k;
+        for (int k = 0; k < (int)fp.t && popc(Rl) < kRegBits; ++k) Rl |= 1u << k;
+
+        FPhase ph{};
+        {
+            int k = 0;
+            for (uint32_t m = Rl; m; m &= m - 1) ph.rpos[k++] = (uint8_t)ctz(m);
+        }
+        for (int i = 0; i < kRegs; ++i) {
+            uint32_t off = 0;
+            for (int k = 0; k < kRegBits; ++k)
+                if (i >> k & 1) off |= 1u << ph.rpos[k];
+            ph.roff[i] = (uint16_t)off;
+        }
+        auto regbit = [&](int q) -> int {
+            const int l = local_of(fp.tpos, fp.t, q);
+            if (l < 0) return -1;
+            for (int k = 0; k < kRegBits; ++k)
+                if (ph.rpos[k] == l) return k;
+            return -1;
+        };
+        auto pattern = [](int j) {  // register indices with bit j set
+            uint32_t p = 0;
+            for (int i = 0; i < kRegs; ++i)
+                if (i >> j & 1) p |= 1u << i;
+            return (uint16_t)p;
+        };
+        ph.op_begin = (uint32_t)prog.ops.size();
+        for (const Prim& p : take) {
+            FOp op{};
+            op.type = p.type;
+            // controls: register / lane / outside-tile
+            uint32_t pc = 0xFFFF;
+            for (uint64_t m = p.ctrl; m; m &= m - 1) {
+                const int q = (int)ctz(m);
+                const int j = regbit(q);
+                const int l = local_of(fp.tpos, fp.t, q);
+                if (j >= 0) pc &= pattern(j);
+                else if (l >= 0) op.lctrl |= (uint16_t)(1u << l);
+                else op.gctrl |= 1ull << q;
+            }
+            op.pc = (uint16_t)pc;
+            if (p.type == OP_D) {
+                auto operand = [&](int q, uint8_t& is_reg, uint16_t& pat, uint16_t& lm, uint64_t& gm) {
+                    if (q < 0) return;  // absent: always 0
+                    const int j = regbit(q);
+                    const int l = local_of(fp.tpos, fp.t, q);
+                    if (j >= 0) { is_reg = 1; pat = pattern(j); }
+                    else if (l >= 0) lm = (uint16_t)(1u << l);
+                    else gm = 1ull << q;
+                };
+                operand(p.q0, op.a_is_reg, op.pa, op.la, op.ga);
+                operand(p.q1, op.b_is_reg, op.pb, op.lb, op.gb);
+                op.coef = add_coef(prog, p.m, 4);
+            } else if (p.type == OP_M2) {
+                op.j0 = (uint8_t)regbit(p.q0);
+                op.coef = add_coef(prog, p.m, 4);
+            } else {
+                int j0 = regbit(p.q0), j1 = regbit(p.q1);
+                cd u[16];
+                if (j0 < j1) {
+                    for (int k = 0; k < 16; ++k) u[k] = p.m[k];
+                } else {  // swap the roles of matrix bits 0 and 1
+                    auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+                    for (int r = 0; r < 4; ++r)
+                        for (int c = 0; c < 4; ++c) u[4 * sw(r) + sw(c)] = p.m[4 * r + c];
+                    std::swap(j0, j1);
+                }
+                op.j0 = (uint8_t)j0;
+                op.j1 = (uint8_t)j1;
+                op.coef = add_coef(prog, u, 16);
+            }
+            prog.ops.push_back(op);
+        }
+        ph.op_end = (uint32_t)prog.ops.size();
+        prog.phases.push_back(ph);
+        rest.swap(leave);
+    }
+    fp.phase_end = (uint32_t)prog.phases.size();
+    if (fp.phase_end == fp.phase_begin) {  // a pass with no primitive cannot happen; keep one empty phase
+        FPhase ph{};
+        for (int k = 0; k < kRegBits; ++k) ph.rpos[k] = (uint8_t)(fp.t - kRegBits + k);
+        for (int i = 0; i < kRegs; ++i) ph.roff[i] = (uint16_t)(i << (fp.t - kRegBits));
+        ph.op_begin = ph.op_end = (uint32_t)prog.ops.size();
+        prog.phases.push_back(ph);
+        fp.phase_end = fp.phase_begin + 1;
+    }
+    auto is_top = [&](const FPhase& ph) {
+        for (int k = 0; k < kRegBits; ++k)
+            if (ph.rpos[k] != fp.t - kRegBits + k) return false;
+        return true;
+    };
+    fp.direct_first = is_top(prog.phases[fp.phase_begin]) ? 1u : 0u;
+    fp.direct_last = is_top(prog.phases[fp.phase_end - 1]) ? 1u : 0u;
+    prog.passes.push_back(fp);
+}
+
+}  // namespace
+
+void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps) {
+    const uint64_t lmask = (1ull << ns) - 1u;
+    size_t i = 0;
+    while (i < n) {
+        // a segment of fusable gates, then (if any) one gate for the plain kernels
+        std::vector<Prim> prims;
+        size_t j = i;
+        for (; j < n; ++j) {
+            if ((g[j].ctrl & ~lmask) || g[j].kind == QVNT_ID) continue;
+            if (!lower(g[j], lmask, prims)) break;
+        }
+        if (!prims.empty()) {
+            Step st;
+            for (const PassPlan& pp : schedule_passes(merge_single(prims), ns, tq)) compile_pass(st.prog, pp);
+            steps.push_back(std::move(st));
+        }
+        if (j < n) {
+            Step st;
+            st.plain = true;
+            st.gate = g[j];
+            steps.push_back(std::move(st));
+            ++j;
+        }
+        i = j;
+    }
+}
+
+}  // namespace fusion
+}  // namespace qvnt

@@ -206,19 +206,11 @@ int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t gro
 
 }  // namespace
 
-int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
-    const uint64_t lmask = r->len - 1u;
+int check_gate(uint64_t len, const qvnt_gate& gate, bool* noop) {
+    const uint64_t lmask = len - 1u;
     const uint32_t k = gate.kind;
+    *noop = false;
     if (k >= QVNT_KIND_COUNT) return fail(QVNT_E_ARG, "unknown gate kind");
-    // A control bit outside the buffer is never set: the predicate
-    // `!idx & ctrl == 0` (dispatch.rs:35) fails everywhere, the gate is the
-    // identity.
-    if (gate.ctrl & ~lmask) return QVNT_OK;
-    if (k == QVNT_ID) return QVNT_OK;
-
-    const bool diag = is_diagonal_kind(k);
-    const bool quad = (k == QVNT_H2 || k == QVNT_U2);
-    const uint64_t touched = touched_bits(gate);
     // Bit-count rules of the checked constructors (atomic/<gate>.rs is_valid).
     switch (k) {
     case QVNT_RX: case QVNT_RY: case QVNT_RZ: case QVNT_H1: case QVNT_U1: case QVNT_PHI:
@@ -234,11 +226,30 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
         break;
     default: break;
     }
+    const uint64_t touched = touched_bits(gate);
     if (gate.ctrl & touched) return fail(QVNT_E_CTRL_OVERLAP, "control mask overlaps the gate's qubits");
+    // A control bit outside the buffer is never set: the predicate
+    // `!idx & ctrl == 0` (dispatch.rs:35) fails everywhere, the gate is the
+    // identity.
+    if ((gate.ctrl & ~lmask) || k == QVNT_ID) {
+        *noop = true;
+        return QVNT_OK;
+    }
     // Non-diagonal gates read psi[idx ^ mask]; beyond the buffer the
     // reference indexes out of bounds (a panic).  Diagonal gates only test
     // idx & mask, which is zero outside the buffer.
-    if (!diag && (touched & ~lmask)) return fail(QVNT_E_ARG, "gate acts on qubits beyond the register");
+    if (!is_diagonal_kind(k) && (touched & ~lmask)) return fail(QVNT_E_ARG, "gate acts on qubits beyond the register");
+    return QVNT_OK;
+}
+
+int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
+    bool noop = false;
+    if (int rc = check_gate(r->len, gate, &noop)) return rc;
+    if (noop) return QVNT_OK;
+    const uint64_t lmask = r->len - 1u;
+    const uint32_t k = gate.kind;
+    const bool diag = is_diagonal_kind(k);
+    const bool quad = (k == QVNT_H2 || k == QVNT_U2);
 
     Deposit d{};
     GClass cls;

@@ -118,6 +118,7 @@ int qvnt_reg_free(qvnt_reg* r) {
     if (r->tree) (void)hipFree(r->tree);
     if (r->dscratch) (void)hipFree(r->dscratch);
     if (r->iscratch) (void)hipFree(r->iscratch);
+    if (r->prog) (void)hipFree(r->prog);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return QVNT_OK;
@@ -130,11 +131,26 @@ qvnt_prec qvnt_reg_prec(const qvnt_reg* r) { return r ? r->prec : QVNT_F64; }
 int qvnt_reg_apply_gates(qvnt_reg* r, const qvnt_gate* gates, size_t count) {
     if (int rc = check_reg(r)) return rc;
     if (count && !gates) return fail(QVNT_E_ARG, "null gate array");
+    // validate the whole queue first: nothing is applied if a gate is bad
+    for (size_t i = 0; i < count; ++i) {
+        bool noop = false;
+        if (int rc = check_gate(r->len, gates[i], &noop)) return rc;
+    }
     for (size_t i = 0; i < count; ++i) {
         int rc = launch_gate(r, gates[i]);
         if (rc) return rc;
     }
     return QVNT_OK;
+}
+
+int qvnt_reg_apply_fused(qvnt_reg* r, const qvnt_op* op, uint32_t max_qubits, uint32_t* passes) {
+    if (int rc = check_reg(r)) return rc;
+    if (!op) return fail(QVNT_E_ARG, "null op");
+    for (const qvnt_gate& g : op->g) {
+        bool noop = false;
+        if (int rc = check_gate(r->len, g, &noop)) return rc;
+    }
+    return apply_fused(r, op->g.data(), op->g.size(), max_qubits, passes);
 }
 
 int qvnt_reg_apply(qvnt_reg* r, const qvnt_op* op) {

@@ -32,6 +32,10 @@ struct qvnt_reg {
     uint64_t* iscratch = nullptr;    // small device scratch (outcomes)
     size_t shots_cap = 0;
 
+    // fused-form program (phases, micro-ops, coefficients) on the device
+    void* prog = nullptr;
+    size_t prog_cap = 0;
+
     // per-kernel-class timing
     bool timing = false;
     struct Pending { int kclass; hipEvent_t a, b; double bytes; };
@@ -55,6 +59,10 @@ struct Timed {
 };
 
 // plain.hip
+// Validates a gate against a buffer of `len` amplitudes (bit counts, control
+// overlap, qubits beyond the buffer); *noop is set for gates that are the
+// identity there (Id, a control bit beyond the buffer).
+int check_gate(uint64_t len, const qvnt_gate& g, bool* noop);
 int launch_gate(qvnt_reg* r, const qvnt_gate& g);
 int launch_init(qvnt_reg* r, uint64_t state);
 
@@ -63,5 +71,8 @@ int tree_build(qvnt_reg* r, uint64_t count, double* total_out);
 int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out);
 int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm);
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n);
+
+// fused.hip
+int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes);
 
 }  // namespace qvnt

@@ -96,6 +96,14 @@ class QReg:
         check(lib.qvnt_reg_apply(self._h, op.handle))
         return self
 
+    def apply_fused(self, op, max_qubits=0):
+        """The fused form: returns the number of passes over the state."""
+        if not isinstance(op, MultiOp):
+            raise TypeError("apply_fused expects a MultiOp")
+        passes = ctypes.c_uint32(0)
+        check(lib.qvnt_reg_apply_fused(self._h, op.handle, max_qubits, ctypes.byref(passes)))
+        return passes.value
+
     def apply_gates(self, gates):
         arr = (type(gates[0]) * len(gates))(*gates) if gates else None
         check(lib.qvnt_reg_apply_gates(self._h, arr, len(gates)))

@@ -1,0 +1,118 @@
+"""The fused form (qvnt_reg_apply_fused) against the CPU oracle.
+
+The fused form reorders gates only past gates on disjoint qubits and
+multiplies one-qubit runs into 2x2 matrices, so it agrees with the
+reference to rounding: the bar is max |dpsi| <= 1e-12 in FP64 and
+infidelity <= 1e-5 in FP32."""
+import sys
+
+import numpy as np
+import pytest
+
+import qvnt_amd
+from qvnt_amd import QReg, circuits, op
+from test_gpu_parity import build, random_circuit
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,tq", [(6, 6), (7, 6), (8, 8), (9, 6), (10, 8), (11, 10), (12, 12), (13, 8),
+                                  (14, 12), (16, 10), (17, 12), (20, 12)])
+def test_random_circuits(orc, n, tq):
+    say("fused random n=%d tq=%d" % (n, tq))
+    g = np.random.default_rng(5000 + 31 * n + tq)
+    specs = random_circuit(g, n, 200)
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    ref = orc.Reg(n, threads=8)
+    ref.set_state(psi)
+    ref.apply(build(specs, orc.op))
+    with QReg(n) as r:
+        r.set_state(psi)
+        passes = r.apply_fused(build(specs, op), max_qubits=tq)
+        got = r.get_state()
+    err = np.max(np.abs(got - ref.state()))
+    say("  passes", passes, "of", len(build(specs, op)), "gates; max |d| = %.2e" % err)
+    assert err <= TOL
+
+
+def test_small_registers_take_the_plain_kernels(orc):
+    for n in range(0, 6):
+        g = np.random.default_rng(n)
+        specs = random_circuit(g, max(n, 1), 40) if n else []
+        ref = orc.Reg(n)
+        ref.apply(build(specs, orc.op))
+        with QReg(n) as r:
+            r.apply_fused(build(specs, op))
+            assert np.array_equal(r.get_state(), ref.state())
+
+
+def test_brickwork_and_measure(orc):
+    n = 20
+    specs = sum((circuits.brickwork_layer(n, l) for l in range(6)), [])
+    ref = orc.Reg(n, threads=8)
+    ref.apply(circuits.build(specs, orc.op))
+    with QReg(n) as r:
+        passes = r.apply_fused(circuits.build(specs, op))
+        got = r.get_state()
+        assert np.max(np.abs(got - ref.state())) <= TOL
+        assert passes * 10 < len(circuits.build(specs, op))
+        # measurement of the fused state follows the contract on that state
+        u = qvnt_amd.uniform(7, 7, 7)
+        want = orc.tree_sample(got, u) & 0xFF0F
+        assert r.measure_mask(0xFF0F, u=u).get() == want
+
+
+def test_plain_20q_fused(orc):
+    state, specs, mask = circuits.plain_20q()
+    ref = orc.Reg(20, state, threads=8)
+    ref.apply(circuits.build(specs, orc.op))
+    with QReg.with_state(20, state) as r:
+        passes = r.apply_fused(circuits.build(specs, op))
+        got = r.get_state()
+    say("plain_20q fused passes", passes)
+    assert np.max(np.abs(got - ref.state())) <= TOL
+
+
+def test_fp32_fused_infidelity(orc):
+    n = 20
+    specs = sum((circuits.brickwork_layer(n, l) for l in range(10)), [])
+    ref = orc.Reg(n, threads=8)
+    ref.apply(circuits.build(specs, orc.op))
+    want = ref.state()
+    with QReg(n, prec="f32") as r:
+        r.apply_fused(circuits.build(specs, op))
+        got = r.get_state().astype(np.complex128)
+    infid = 1.0 - abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    err = np.max(np.abs(got - want))
+    say("fp32 fused: max err %.2e infidelity %.2e" % (err, infid))
+    assert infid <= 1e-5 and err < 1e-5
+
+
+def test_fused_30q_round_trip():
+    n = 30
+    say("fused round trip n=30")
+    fwd = sum((circuits.brickwork_layer(n, l) for l in range(4)), [])
+    inv = []
+    for ctor, args, ctrl in reversed(fwd):
+        if ctor == "u3":
+            th, ph, la, m = args
+            inv += [("rz", (-ph, m), ctrl), ("ry", (-th, m), ctrl), ("rz", (-la, m), ctrl)]
+        elif ctor == "rzz":
+            inv.append(("rzz", (-args[0], args[1]), ctrl))
+        else:
+            inv.append((ctor, args, ctrl))
+    start = 0x15555555
+    with QReg.with_state(n, start) as r:
+        p1 = r.apply_fused(circuits.build(fwd, op))
+        assert abs(r.get_absolute() - 1.0) < 1e-12
+        p2 = r.apply_fused(circuits.build(inv, op))
+        outs = r.sample_mask((1 << n) - 1, [qvnt_amd.uniform(1, 2, k) for k in range(16)])
+        say("passes", p1, p2)
+        assert all(int(o) == start for o in outs)
