@@ -1,0 +1,99 @@
+// The reference's own unit tests and doctests, restated against the C++
+// facade (include/qvnt.hpp) over the device library.  Mode "host" runs the
+// operator tests (no GPU needed); mode "device" runs the register tests.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "qvnt.hpp"
+
+using namespace qvnt;
+static int failures = 0;
+#define EXPECT(c) do { if (!(c)) { std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); ++failures; } } while (0)
+
+// src/operator/single/mod.rs:142-152 (name)
+static void single_name() {
+    MultiOp o = op::x(123);
+    EXPECT(o.name() == "[X123]");
+    EXPECT(o.c(4)->name() == "[C4_X123]");
+}
+// src/operator/single/mod.rs:154-161 (wrong_ctrl_mask)
+static void wrong_ctrl_mask() {
+    MultiOp o = op::ryy(1.35, 0b101);
+    EXPECT((o.act_on() & 0b001) != 0);
+    EXPECT(!o.c(0b001).has_value());
+    EXPECT((o.act_on() & 0b010) == 0);
+    EXPECT(o.c(0b010).has_value());
+}
+// src/operator/multi/mod.rs:201-208 (ops)
+static void multi_ops() {
+    MultiOp pend = op::id() * *op::h(0b001).c(0b010) * *op::x(0b011).c(0b100) * op::rz(5.0, 0b001);
+    EXPECT(pend.len() == 3);
+}
+// README.md:95-98 (should_panic: overlapping control)
+static void readme_overlap() { EXPECT(!op::x(0b001).c(0b001).has_value()); }
+// src/operator/mod.rs:151-156 (expect on a wrong bit count)
+static void rotation_mask_panics() {
+    bool threw = false;
+    try { op::rx(1.0, 0b11); } catch (const Error& e) { threw = e.code == QVNT_E_INVALID_MASK; }
+    EXPECT(threw);
+}
+// src/register/quant.rs:642-677 (quantum_reg)
+static void quantum_reg() {
+    QReg reg = QReg::with_state(4, 0b1100);
+    const uint64_t mask = 0b0110;
+    MultiOp operator_ = op::h(0b1111) * *op::h(0b0011).c(0b1000) * *op::swap(0b1001).c(0b0010);
+    reg.apply(operator_);
+    EXPECT(operator_.name() == "[H3, H12, C8_H3, C2_SWAP9]");
+    const double want[16] = {0.25, 0.25, 0.25, 0.0, -0.25, -0.25, -0.25, 0.0, -0.5, 0.0, 0.25, 0.0, 0.5, 0.0, -0.25, 0.0};
+    auto psi = reg.get_state();
+    for (int i = 0; i < 16; ++i) EXPECT(psi[i] == std::complex<double>(want[i], 0.0));
+    EXPECT((reg.measure_mask(mask).get() & ~mask) == 0);
+}
+// src/register/quant.rs:81-96 (doctest: Bell state)
+static void bell() {
+    QReg q = QReg::new_(2);
+    MultiOp gate = op::h(0b01) * *op::x(0b10).c(0b01);
+    q.apply(gate);
+    auto p = q.get_probabilities();
+    EXPECT(p.size() == 4 && p[0] == 0.5 && p[1] == 0.0 && p[2] == 0.0 && p[3] == 0.5);
+}
+// src/register/quant.rs:679-711 (tensor, the reg3 part)
+static void tensor_reg3() {
+    QReg reg3 = QReg::with_state(3, 0b101);
+    reg3.apply(op::h(0b101));
+    auto p = reg3.get_probabilities();
+    const double want[8] = {0.25, 0.25, 0, 0, 0.25, 0.25, 0, 0};
+    for (int i = 0; i < 8; ++i) EXPECT(std::fabs(p[i] - want[i]) < 1e-9);
+}
+// benches/performance.rs:4-11 (perf_test_single) at 18..20 qubits, plain and fused
+static void bench_workload() {
+    for (uint32_t q = 18; q <= 20; ++q) {
+        QReg reg = QReg::with_state(q, 0);
+        reg.apply(op::qft(0b0111) * op::qft(0b1110));
+        const uint64_t mask = 0b100;
+        EXPECT((reg.measure_mask(mask).get() & ~mask) == 0);
+        QReg f = QReg::with_state(q, 0);
+        f.apply_fused(op::qft(0b0111) * op::qft(0b1110));
+        EXPECT(std::fabs(f.get_absolute() - 1.0) < 1e-12);
+    }
+}
+
+int main(int argc, char** argv) {
+    const std::string mode = argc > 1 ? argv[1] : "host";
+    single_name();
+    wrong_ctrl_mask();
+    multi_ops();
+    readme_overlap();
+    rotation_mask_panics();
+    if (mode == "device") {
+        quantum_reg();
+        bell();
+        tensor_reg3();
+        bench_workload();
+    }
+    std::printf("%s: %d failure(s)\n", mode.c_str(), failures);
+    return failures ? 1 : 0;
+}
