@@ -87,10 +87,11 @@ def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value):
         reg.close()
         v = timed_gates / wall
         ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] else 0.0
-        log("fused (%s): %d passes for %d SingleOps in %.3f s -> %.1f gates/s (x%.1f plain); k_fused %.1f GB/s; |psi|^2-1 = %.1e"
-            % (label, passes, timed_gates, wall, v, v / plain_value, ach, norm - 1.0))
+        speedup = v / plain_value if plain_value else None
+        log("fused (%s): %d passes for %d SingleOps in %.3f s -> %.1f gates/s (x%s plain); k_fused %.1f GB/s; |psi|^2-1 = %.1e"
+            % (label, passes, timed_gates, wall, v, "%.1f" % speedup if speedup else "?", ach, norm - 1.0))
         out[label] = {"value": round(v, 3), "passes": passes, "seconds": round(wall, 4),
-                      "speedup_vs_plain": round(v / plain_value, 2),
+                      "speedup_vs_plain": round(speedup, 2) if speedup else None,
                       "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_fused",
                                    "launches": st["launches"],
@@ -118,6 +119,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -131,6 +133,11 @@ def main():
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]
     ops = [circuits.build(s, op) for s in layers]
     timed_gates = sum(len(o) for o in ops[args.warmup:])
+
+    if args.no_plain:
+        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, None)
+        print(json.dumps({"fused": fused}), flush=True)
+        return
 
     reg = QReg(n)
     log("register: %d qubits FP64, %.1f GiB" % (n, (16 << n) / 2**30))

@@ -23,30 +23,37 @@ namespace {
 
 using namespace fusion;
 
-__device__ __forceinline__ uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
+// the LDS swizzle of fusion.hpp (lds_swizzle), for device code
+__device__ __forceinline__ uint32_t dswz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
 
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// complex multiply(-add) as explicit FMA chains: 4 instructions each
 template <class R>
 __device__ __forceinline__ cplx<R> fma_c(cplx<R> acc, cplx<R> a, cplx<R> x) {
-    acc.re += a.re * x.re - a.im * x.im;
-    acc.im += a.re * x.im + a.im * x.re;
+    acc.re = fmar(-a.im, x.im, fmar(a.re, x.re, acc.re));
+    acc.im = fmar(a.im, x.re, fmar(a.re, x.im, acc.im));
     return acc;
 }
 template <class R>
 __device__ __forceinline__ cplx<R> mul_c(cplx<R> a, cplx<R> x) {
-    return {a.re * x.re - a.im * x.im, a.re * x.im + a.im * x.re};
+    return {fmar(a.re, x.re, -(a.im * x.im)), fmar(a.re, x.im, a.im * x.re)};
 }
 template <class R>
-__device__ __forceinline__ cplx<R> ld_coef(const double* c, int k) {
+__device__ __forceinline__ cplx<R> ld_coef(const double* __restrict__ c, int k) {
     return {(R)c[2 * k], (R)c[2 * k + 1]};
 }
 
-template <int J, class R>
-__device__ __forceinline__ void apply_m2(cplx<R>* v, const double* c, uint32_t pc) {
+// 2x2 on register bit J for the register indices in the uniform pattern
+// pc (register controls); MASKED = false when there are none.
+template <int J, bool MASKED, class R>
+__device__ __forceinline__ void apply_m2(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
     const cplx<R> u00 = ld_coef<R>(c, 0), u01 = ld_coef<R>(c, 1), u10 = ld_coef<R>(c, 2), u11 = ld_coef<R>(c, 3);
 #pragma unroll
     for (int i = 0; i < kRegs; ++i) {
         if (i & (1 << J)) continue;
-        if (!((pc >> i) & 1u)) continue;
+        if (MASKED && !((pc >> i) & 1u)) continue;
         const cplx<R> x0 = v[i], x1 = v[i | (1 << J)];
         v[i] = fma_c(mul_c(u00, x0), u01, x1);
         v[i | (1 << J)] = fma_c(mul_c(u10, x0), u11, x1);
@@ -54,7 +61,7 @@ __device__ __forceinline__ void apply_m2(cplx<R>* v, const double* c, uint32_t p
 }
 
 template <int J0, int J1, class R>
-__device__ __forceinline__ void apply_m4(cplx<R>* v, const double* c, uint32_t pc) {
+__device__ __forceinline__ void apply_m4(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
 #pragma unroll
     for (int i = 0; i < kRegs; ++i) {
         if (i & ((1 << J0) | (1 << J1))) continue;
@@ -74,16 +81,11 @@ __device__ __forceinline__ void apply_m4(cplx<R>* v, const double* c, uint32_t p
     }
 }
 
+// diagonal: one phase per register (host tables, controls folded in)
 template <class R>
-__device__ __forceinline__ void apply_d(cplx<R>* v, const double* c, uint32_t pc, uint32_t pa, uint32_t pb) {
-    const cplx<R> t0 = ld_coef<R>(c, 0), t1 = ld_coef<R>(c, 1), t2 = ld_coef<R>(c, 2), t3 = ld_coef<R>(c, 3);
+__device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c) {
 #pragma unroll
-    for (int i = 0; i < kRegs; ++i) {
-        if (!((pc >> i) & 1u)) continue;
-        const bool a = (pa >> i) & 1u, b = (pb >> i) & 1u;
-        const cplx<R> e = b ? (a ? t3 : t2) : (a ? t1 : t0);
-        v[i] = mul_c(e, v[i]);
-    }
+    for (int i = 0; i < kRegs; ++i) v[i] = mul_c(ld_coef<R>(c, i), v[i]);
 }
 
 template <class R>
@@ -91,22 +93,39 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
                                           const double* __restrict__ coef, uint32_t tb, uint64_t tile) {
     for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
         const FOp& op = ops[o];
-        // per-thread control: lane bits of this thread's base, bits of the tile
+        // controls on lane bits (this thread's base) and on bits outside the tile
         if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
         const double* c = coef + op.coef;
         if (op.type == OP_D) {
-            const uint32_t pa = op.a_is_reg ? op.pa : (((tb & op.la) | (tile & op.ga)) ? 0xFFFFu : 0u);
-            const uint32_t pb = op.b_is_reg ? op.pb : (((tb & op.lb) | (tile & op.gb)) ? 0xFFFFu : 0u);
-            apply_d(v, c, op.pc, pa, pb);
+            // table variant from the non-register operands; a waterfall over
+            // the variants present in the wave keeps the table address uniform
+            const uint32_t k = ((((tb & op.la) | (tile & op.ga)) != 0) ? 1u : 0u) |
+                               ((((tb & op.lb) | (tile & op.gb)) != 0) ? 2u : 0u);
+            for (;;) {
+                const uint32_t kk = __builtin_amdgcn_readfirstlane(k);
+                if (k == kk) {
+                    apply_d(v, c + 2 * kRegs * kk);
+                    break;
+                }
+            }
         } else if (op.type == OP_M2) {
-            switch (op.j0) {
-            case 0: apply_m2<0>(v, c, op.pc); break;
-            case 1: apply_m2<1>(v, c, op.pc); break;
-            case 2: apply_m2<2>(v, c, op.pc); break;
-            default: apply_m2<3>(v, c, op.pc); break;
+            if (op.pc == 0xFFFFu) {
+                switch (op.sel) {
+                case 0: apply_m2<0, false>(v, c, 0u); break;
+                case 1: apply_m2<1, false>(v, c, 0u); break;
+                case 2: apply_m2<2, false>(v, c, 0u); break;
+                default: apply_m2<3, false>(v, c, 0u); break;
+                }
+            } else {
+                switch (op.sel) {
+                case 0: apply_m2<0, true>(v, c, op.pc); break;
+                case 1: apply_m2<1, true>(v, c, op.pc); break;
+                case 2: apply_m2<2, true>(v, c, op.pc); break;
+                default: apply_m2<3, true>(v, c, op.pc); break;
+                }
             }
         } else {
-            switch (op.j0 * 4 + op.j1) {
+            switch (op.sel) {
             case 1: apply_m4<0, 1>(v, c, op.pc); break;
             case 2: apply_m4<0, 2>(v, c, op.pc); break;
             case 3: apply_m4<0, 3>(v, c, op.pc); break;
@@ -134,11 +153,14 @@ __global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass 
     // global offset of this thread's local bits 0..t-5
     uint64_t lo_off = 0;
     for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
+    const uint64_t base = tile | lo_off;
 
     cplx<R> v[kRegs];
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) v[j] = psi[tile | lo_off | pass.hoff[j]];
+    for (int j = 0; j < kRegs; ++j) v[j] = psi[base + pass.hoff[j]];
 
+    // the direct layout: register j <-> local tid + j * nth
+    const uint32_t sdir = dswz(tid);
     const uint32_t p0 = pass.phase_begin, p1 = pass.phase_end;
     for (uint32_t p = p0; p < p1; ++p) {
         const FPhase& ph = phases[p];
@@ -149,30 +171,31 @@ __global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass 
             const uint32_t lo = tb & ((1u << ph.rpos[k]) - 1u);
             tb = ((tb ^ lo) << 1) | lo;
         }
+        const uint32_t stb = dswz(tb);
         const bool direct_in = (p == p0) && pass.direct_first;
         const bool direct_out = (p + 1 == p1) && pass.direct_last;
         if (!direct_in) {
             if (p == p0) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) lds[swz(tid + j * nth)] = v[j];
+                for (int j = 0; j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
                 __syncthreads();
             }
 #pragma unroll
-            for (int i = 0; i < kRegs; ++i) v[i] = lds[swz(tb | ph.roff[i])];
+            for (int i = 0; i < kRegs; ++i) v[i] = lds[stb ^ ph.sroff[i]];
         }
         run_phase(v, ph, ops, coef, tb, tile);
         if (!direct_out) {
 #pragma unroll
-            for (int i = 0; i < kRegs; ++i) lds[swz(tb | ph.roff[i])] = v[i];
+            for (int i = 0; i < kRegs; ++i) lds[stb ^ ph.sroff[i]] = v[i];
             __syncthreads();
             if (p + 1 == p1) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) v[j] = lds[swz(tid + j * nth)];
+                for (int j = 0; j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
             }
         }
     }
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) psi[tile | lo_off | pass.hoff[j]] = v[j];
+    for (int j = 0; j < kRegs; ++j) psi[base + pass.hoff[j]] = v[j];
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }

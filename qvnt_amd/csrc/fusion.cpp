@@ -247,9 +247,9 @@ std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsig
 }
 
 
-int local_of(const uint8_t* tpos, unsigned t, int q) {
+int local_of(const uint32_t* tpos, unsigned t, int q) {
     for (unsigned k = 0; k < t; ++k)
-        if (tpos[k] == q) return (int)k;
+        if ((int)tpos[k] == q) return (int)k;
     return -1;
 }
 
@@ -262,13 +262,26 @@ uint32_t add_coef(Program& prog, const cd* m, int n) {
     return off;
 }
 
+FPhase make_phase(uint32_t Rl) {
+    FPhase ph{};
+    int k = 0;
+    for (uint32_t m = Rl; m; m &= m - 1) ph.rpos[k++] = ctz(m);
+    for (int i = 0; i < kRegs; ++i) {
+        uint32_t off = 0;
+        for (int b = 0; b < kRegBits; ++b)
+            if (i >> b & 1) off |= 1u << ph.rpos[b];
+        ph.sroff[i] = lds_swizzle(off);
+    }
+    return ph;
+}
+
 void compile_pass(Program& prog, const PassPlan& pp) {
     FPass fp{};
     fp.t = popc(pp.T);
     fp.nthreads = 1u << (fp.t - kRegBits);
     {
         unsigned k = 0;
-        for (uint64_t m = pp.T; m; m &= m - 1) fp.tpos[k++] = (uint8_t)ctz(m);
+        for (uint64_t m = pp.T; m; m &= m - 1) fp.tpos[k++] = ctz(m);
     }
     for (int j = 0; j < kRegs; ++j) {
         uint64_t off = 0;
@@ -296,62 +309,58 @@ void compile_pass(Program& prog, const PassPlan& pp) {
         }
         // complete the register set, preferring the top local bits (direct HBM access)
         for (int k = (int)fp.t - 1; k >= 0 && popc(Rl) < kRegBits; --k)
-            if (top_regs >> k & 1) Rl |= 1u << k;
+            if ((top_regs >> k) & 1) Rl |= 1u << k;
         for (int k = 0; k < (int)fp.t && popc(Rl) < kRegBits; ++k) Rl |= 1u << k;
 
-        FPhase ph{};
-        {
-            int k = 0;
-            for (uint32_t m = Rl; m; m &= m - 1) ph.rpos[k++] = (uint8_t)ctz(m);
-        }
-        for (int i = 0; i < kRegs; ++i) {
-            uint32_t off = 0;
-            for (int k = 0; k < kRegBits; ++k)
-                if (i >> k & 1) off |= 1u << ph.rpos[k];
-            ph.roff[i] = (uint16_t)off;
-        }
+        FPhase ph = make_phase(Rl);
         auto regbit = [&](int q) -> int {
             const int l = local_of(fp.tpos, fp.t, q);
             if (l < 0) return -1;
             for (int k = 0; k < kRegBits; ++k)
-                if (ph.rpos[k] == l) return k;
+                if ((int)ph.rpos[k] == l) return k;
             return -1;
-        };
-        auto pattern = [](int j) {  // register indices with bit j set
-            uint32_t p = 0;
-            for (int i = 0; i < kRegs; ++i)
-                if (i >> j & 1) p |= 1u << i;
-            return (uint16_t)p;
         };
         ph.op_begin = (uint32_t)prog.ops.size();
         for (const Prim& p : take) {
             FOp op{};
             op.type = p.type;
             // controls: register / lane / outside-tile
-            uint32_t pc = 0xFFFF;
+            uint32_t rctrl = 0;  // register bits that must be 1
             for (uint64_t m = p.ctrl; m; m &= m - 1) {
                 const int q = (int)ctz(m);
                 const int j = regbit(q);
                 const int l = local_of(fp.tpos, fp.t, q);
-                if (j >= 0) pc &= pattern(j);
-                else if (l >= 0) op.lctrl |= (uint16_t)(1u << l);
+                if (j >= 0) rctrl |= 1u << j;
+                else if (l >= 0) op.lctrl |= 1u << l;
                 else op.gctrl |= 1ull << q;
             }
-            op.pc = (uint16_t)pc;
+            uint32_t pc = 0;
+            for (int i = 0; i < kRegs; ++i)
+                if (((uint32_t)i & rctrl) == rctrl) pc |= 1u << i;
+            op.pc = pc;
             if (p.type == OP_D) {
-                auto operand = [&](int q, uint8_t& is_reg, uint16_t& pat, uint16_t& lm, uint64_t& gm) {
-                    if (q < 0) return;  // absent: always 0
-                    const int j = regbit(q);
+                // operand a / b: register bit, lane bit, bit outside the tile, or absent
+                int ja = -1, jb = -1;
+                auto operand = [&](int q, int& j, uint32_t& lm, uint64_t& gm) {
+                    if (q < 0) return;
+                    j = regbit(q);
+                    if (j >= 0) return;
                     const int l = local_of(fp.tpos, fp.t, q);
-                    if (j >= 0) { is_reg = 1; pat = pattern(j); }
-                    else if (l >= 0) lm = (uint16_t)(1u << l);
+                    if (l >= 0) lm = 1u << l;
                     else gm = 1ull << q;
                 };
-                operand(p.q0, op.a_is_reg, op.pa, op.la, op.ga);
-                operand(p.q1, op.b_is_reg, op.pb, op.lb, op.gb);
-                op.coef = add_coef(prog, p.m, 4);
+                operand(p.q0, ja, op.la, op.ga);
+                operand(p.q1, jb, op.lb, op.gb);
+                cd tab[4 * kRegs];
+                for (int k = 0; k < 4; ++k)
+                    for (int i = 0; i < kRegs; ++i) {
+                        const int A = ja >= 0 ? (i >> ja) & 1 : (k & 1);
+                        const int B = jb >= 0 ? (i >> jb) & 1 : (k >> 1);
+                        tab[k * kRegs + i] = ((pc >> i) & 1u) ? p.m[A | (B << 1)] : cd(1.0, 0.0);
+                    }
+                op.coef = add_coef(prog, tab, 4 * kRegs);
             } else if (p.type == OP_M2) {
-                op.j0 = (uint8_t)regbit(p.q0);
+                op.sel = (uint32_t)regbit(p.q0);
                 op.coef = add_coef(prog, p.m, 4);
             } else {
                 int j0 = regbit(p.q0), j1 = regbit(p.q1);
@@ -364,8 +373,7 @@ void compile_pass(Program& prog, const PassPlan& pp) {
                         for (int c = 0; c < 4; ++c) u[4 * sw(r) + sw(c)] = p.m[4 * r + c];
                     std::swap(j0, j1);
                 }
-                op.j0 = (uint8_t)j0;
-                op.j1 = (uint8_t)j1;
+                op.sel = (uint32_t)(4 * j0 + j1);
                 op.coef = add_coef(prog, u, 16);
             }
             prog.ops.push_back(op);
@@ -375,14 +383,6 @@ void compile_pass(Program& prog, const PassPlan& pp) {
         rest.swap(leave);
     }
     fp.phase_end = (uint32_t)prog.phases.size();
-    if (fp.phase_end == fp.phase_begin) {  // a pass with no primitive cannot happen; keep one empty phase
-        FPhase ph{};
-        for (int k = 0; k < kRegBits; ++k) ph.rpos[k] = (uint8_t)(fp.t - kRegBits + k);
-        for (int i = 0; i < kRegs; ++i) ph.roff[i] = (uint16_t)(i << (fp.t - kRegBits));
-        ph.op_begin = ph.op_end = (uint32_t)prog.ops.size();
-        prog.phases.push_back(ph);
-        fp.phase_end = fp.phase_begin + 1;
-    }
     auto is_top = [&](const FPhase& ph) {
         for (int k = 0; k < kRegBits; ++k)
             if (ph.rpos[k] != fp.t - kRegBits + k) return false;

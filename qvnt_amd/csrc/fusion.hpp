@@ -34,26 +34,28 @@ constexpr int kMinFused = 6;               // below 2^6 amplitudes the plain ker
 // ---------------------------------------------------------- the program
 enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2 };
 
-struct FOp {
-    uint8_t type;
-    uint8_t j0, j1;        // register bits of the matrix qubits (M4: j0 < j1)
-    uint8_t a_is_reg, b_is_reg;
-    uint8_t pad[1];
-    uint16_t pc;           // register-index pattern satisfying the register controls
-    uint16_t pa, pb;       // D: bit patterns of operands a / b over register indices
-    uint16_t lctrl;        // controls on lane (non-register tile) bits, local positions
-    uint16_t la, lb;       // D: operand a / b as lane bits (local positions)
-    uint16_t pad2;
-    uint64_t gctrl;        // controls on bits outside the tile (global positions)
-    uint64_t ga, gb;       // D: operands outside the tile (global positions)
-    uint32_t coef;         // offset (in doubles) into the coefficient array
-    uint32_t pad3;
+// All program fields are 32/64-bit and uniform across a block, so the
+// kernel reads them with scalar loads into SGPRs.
+struct FOp {                // 64 bytes
+    uint32_t type;          // OP_M2 / OP_M4 / OP_D
+    uint32_t sel;           // M2: register bit j0; M4: 4*j0 + j1 (j0 < j1)
+    uint32_t pc;            // M2/M4: register indices that satisfy the register controls (0xFFFF: all)
+    uint32_t lctrl;         // controls on lane bits (local positions)
+    uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)
+    uint32_t coef;          // offset (in doubles) into the coefficient array
+    uint32_t pad;
+    uint64_t gctrl;         // controls on bits outside the tile (global positions)
+    uint64_t ga, gb;        // D: operand a / b when it lies outside the tile
+    uint64_t pad2;
 };
+// D coefficients: 4 variants (k = [a is a non-register 1] | 2 [b is a
+// non-register 1]) of the 16 per-register phases, register controls
+// folded in (phase 1 where they fail).  M2: 4 complex; M4: 16 complex.
 
 struct FPhase {
     uint32_t op_begin, op_end;
-    uint8_t rpos[kRegBits];   // local positions of the register bits, ascending
-    uint16_t roff[kRegs];     // local index offset of register i
+    uint32_t rpos[kRegBits];  // local positions of the register bits, ascending
+    uint32_t sroff[kRegs];    // swizzled LDS offset of register i (the swizzle is XOR-linear)
 };
 
 struct FPass {
@@ -62,9 +64,13 @@ struct FPass {
     uint32_t phase_begin, phase_end;
     uint32_t direct_first;    // 1: the first phase's registers are local bits t-4..t-1
     uint32_t direct_last;     // 1: same for the last phase
-    uint8_t tpos[kMaxTile];   // global positions of the local bits, ascending
+    uint32_t tpos[kMaxTile];  // global positions of the local bits, ascending
     uint64_t hoff[kRegs];     // global offset of local bits t-4..t-1 = j
 };
+
+// LDS swizzle: bits 4..7 of the local index are XORed into bits 0..3, so
+// 16 lanes that differ in bits 4..7 hit 16 different 16-byte bank groups.
+inline uint32_t lds_swizzle(uint32_t e) { return e ^ ((e >> 4) & 15u); }
 
 struct Program {
     std::vector<FPass> passes;
