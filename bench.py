@@ -100,6 +100,94 @@ def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value):
     return out
 
 
+def tfim(args):
+    """Transverse-field Ising Trotter steps on a ring (rzz on every edge, then
+    rx on every qubit; J = 1, dt = 0.05, h_i ~ U[0.5, 1.5]); a step is one
+    Trotter step of 2n SingleOps.  The plain form runs --steps steps; the
+    fused form runs all --fused-steps steps, one call per step and then the
+    whole evolution as one call."""
+    from qvnt_amd import QReg, circuits, op
+    n, prec = args.qubits, args.prec
+    step = circuits.build(circuits.tfim_step(n), op)
+    amp = 16 if prec == "f64" else 8
+    out = {"plain": None, "fused": {}}
+    log("tfim: %d qubits %s (%.1f GiB)" % (n, prec, (amp << n) / 2**30))
+    if args.steps > 0 and not args.no_plain:
+        reg = QReg(n, prec=prec)
+        reg.set_timing(True)
+        reg.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            reg.apply(step)
+        reg.sync()
+        wall = time.perf_counter() - t0
+        st = reg.kernel_stats(reset=True)
+        norm = reg.get_absolute()
+        reg.close()
+        gates = args.steps * len(step)
+        dom = max((k for k in st if st[k]["launches"]), key=lambda k: st[k]["ms"])
+        d = st[dom]
+        ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+        out["plain"] = {"value": round(gates / wall, 3), "steps": args.steps, "seconds": round(wall, 3),
+                        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": dom,
+                                     "avg_ms": round(d["ms"] / d["launches"], 3)}}
+        log("tfim plain: %d SingleOps in %.2f s -> %.2f gates/s, %s %.0f GB/s; |psi|^2-1 = %.1e"
+            % (gates, wall, gates / wall, dom, ach, norm - 1.0))
+    for label, per_call in (("per_step", 1), ("whole", args.fused_steps)):
+        full = step.clone()
+        for _ in range(per_call - 1):
+            full *= step
+        reg = QReg(n, prec=prec)
+        reg.set_timing(True)
+        reg.sync()
+        passes = 0
+        t0 = time.perf_counter()
+        for _ in range(args.fused_steps // per_call):
+            passes += reg.apply_fused(full)
+        reg.sync()
+        wall = time.perf_counter() - t0
+        st = reg.kernel_stats(reset=True)["fused"]
+        norm = reg.get_absolute()
+        reg.close()
+        gates = args.fused_steps * len(step)
+        ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+        out["fused"][label] = {"value": round(gates / wall, 3), "passes": passes, "seconds": round(wall, 3),
+                               "k_fused_gbs": round(ach, 1), "norm_minus_1": norm - 1.0}
+        log("tfim fused (%s): %d steps, %d passes, %.2f s -> %.1f gates/s, k_fused %.0f GB/s; |psi|^2-1 = %.1e"
+            % (label, args.fused_steps, passes, wall, gates / wall, ach, norm - 1.0))
+    line = {"metric": METRIC, "value": out["plain"]["value"] if out["plain"] else None, "unit": "gates/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": prec, "data": "synthetic (seeded TFIM ring)",
+            "config": {"workload": "tfim-ring-%dq-%s" % (n, prec), "qubits": n, "state_gib": (amp << n) / 2**30,
+                       "fused_steps": args.fused_steps},
+            "plain": out["plain"], "fused": out["fused"]}
+    print(json.dumps(line), flush=True)
+
+
+def tfim_pair(args):
+    """The same TFIM evolution in an FP64 and an FP32 register side by side;
+    after every step the device compares them (max |dpsi|, infidelity)."""
+    from qvnt_amd import QReg, circuits, op
+    n = args.qubits
+    step = circuits.build(circuits.tfim_step(n), op)
+    r64, r32 = QReg(n, prec="f64"), QReg(n, prec="f32")
+    rows = []
+    for k in range(args.fused_steps):
+        if args.no_fused:
+            r64.apply(step)
+            r32.apply(step)
+        else:
+            r64.apply_fused(step)
+            r32.apply_fused(step)
+        err, infid, _, n64, n32 = r64.compare(r32)
+        rows.append({"step": k + 1, "max_abs_err": err, "infidelity": infid, "norm64_minus_1": n64 - 1.0,
+                     "norm32_minus_1": n32 - 1.0})
+        log("step %2d: max|dpsi| %.3e  infidelity %.3e" % (k + 1, err, infid))
+    print(json.dumps({"config": {"workload": "tfim-ring-%dq fp32 vs fp64" % n, "form": "plain" if args.no_fused else "fused"},
+                      "steps": rows}), flush=True)
+
+
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -120,6 +208,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair"])
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--fused-steps", type=int, default=50, help="tfim: Trotter steps through the fused form")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -128,6 +219,11 @@ def main():
 
     import qvnt_amd
     from qvnt_amd import QReg, circuits, op
+
+    if args.workload == "tfim":
+        return tfim(args)
+    if args.workload == "tfim-pair":
+        return tfim_pair(args)
 
     n = args.qubits
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]

@@ -209,6 +209,12 @@ int qvnt_reg_measure(qvnt_reg* reg, double u, uint32_t flags, uint64_t* out);
  * (each equals measure_mask on a copy of the register). */
 int qvnt_reg_sample_mask(qvnt_reg* reg, uint64_t mask, const double* u, size_t shots, uint64_t* out);
 
+/* Device-side comparison of two registers of equal size (any precisions;
+ * no reference counterpart): out = { Re<a|b>, Im<a|b>, |a|^2, |b|^2,
+ * max_i |a_i - b_i| }, accumulated in FP64.  Infidelity is
+ * 1 - |<a|b>|^2 / (|a|^2 |b|^2).  Synchronises both registers. */
+int qvnt_reg_compare(qvnt_reg* a, qvnt_reg* b, double out[5]);
+
 /* Copy-out / copy-in of the interleaved amplitudes (bytes = len * 2 *
  * sizeof(double or float)).  Blocking. */
 int qvnt_reg_get_state(qvnt_reg* reg, void* dst, size_t bytes);

@@ -1,6 +1,6 @@
 """ctypes binding of libqvnt_hip.so (the C ABI declared in include/qvnt.h).
 
-The library is built in-tree (``python -m qvnt_amd.build``).  There is no
+The library is built in-tree (``python qvnt_amd/build.py``).  There is no
 fallback: if the library is missing or cannot be loaded, importing the
 package raises.
 """
@@ -92,6 +92,7 @@ def _declare(L):
         "qvnt_reg_measure_mask": ([vp, u64, d, u32, u64p], i),
         "qvnt_reg_measure": ([vp, d, u32, u64p], i),
         "qvnt_reg_sample_mask": ([vp, u64, dp, sz, u64p], i),
+        "qvnt_reg_compare": ([vp, vp, dp], i),
         "qvnt_reg_get_state": ([vp, vp, sz], i),
         "qvnt_reg_set_state": ([vp, vp, sz], i),
         "qvnt_reg_sync": ([vp], i),
@@ -110,7 +111,7 @@ def _declare(L):
 
 def load():
     if not os.path.exists(LIB_PATH):
-        raise ImportError("qvnt_amd: %s is not built (run `python -m qvnt_amd.build`); "
+        raise ImportError("qvnt_amd: %s is not built (run `python qvnt_amd/build.py`); "
                           "there is no CPU fallback" % LIB_PATH)
     return _declare(ctypes.CDLL(LIB_PATH))
 

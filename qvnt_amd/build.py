@@ -1,6 +1,7 @@
 """Builds libqvnt_hip.so in-tree with hipcc for gfx950.
 
-Usage: python -m qvnt_amd.build  (or __graft_entry__.build()).
+Usage: python qvnt_amd/build.py  (or __graft_entry__.build()); it does not
+import the package, which loads the library it builds.
 Objects go to qvnt_amd/_build/, the library to qvnt_amd/lib/.
 """
 import concurrent.futures

@@ -14,6 +14,7 @@
 // reduced, searched and rescaled in FP64 as well.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -36,10 +37,15 @@ __device__ __forceinline__ double weight(cplx<R> p) {
 // node(11, blockIdx.x): each wave reduces 512 consecutive leaves, 64 per
 // load; lanes pair up with xor-shuffles (adjacent pairs first), then the 8
 // chunk sums and the 4 wave sums pair up in the same adjacent order.
+// A dispatch carries its size in work-items as a 32-bit number: grids are
+// capped at kMaxGrid blocks and kernels stride over their work.
+constexpr uint64_t kMaxGrid = 1u << 20;
+
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_leaf(const cplx<R>* __restrict__ psi, double* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_leaf(const cplx<R>* __restrict__ psi, uint64_t nleaves, double* __restrict__ out) {
+  for (uint64_t blk = blockIdx.x; blk < nleaves; blk += gridDim.x) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t base = (uint64_t)blockIdx.x * kLeaf + (uint64_t)wave * 512u;
+    const uint64_t base = blk * kLeaf + (uint64_t)wave * 512u;
     double s[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) s[j] = weight(psi[base + (uint64_t)j * 64u + (uint64_t)lane]);
@@ -51,7 +57,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf(const cplx<R>* __restrict__ psi
     __shared__ double sh[4];
     if (lane == 0) sh[wave] = w;
     __syncthreads();
-    if (threadIdx.x == 0) out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    if (threadIdx.x == 0) out[blk] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+  }
 }
 
 // Pairwise levels of `span` (<= 2048, a power of two) values held in LDS:
@@ -154,19 +162,19 @@ __global__ __launch_bounds__(kBlock) void k_sample(const cplx<R>* __restrict__ p
 // (idx ^ r) & mask != 0; store-only.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_collapse(cplx<R>* __restrict__ psi, uint64_t len, uint64_t r, uint64_t mask) {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < len && ((i ^ r) & mask)) psi[i] = cplx<R>{(R)0, (R)0};
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (uint64_t)gridDim.x * kBlock)
+        if ((i ^ r) & mask) psi[i] = cplx<R>{(R)0, (R)0};
 }
 
 // Renormalisation of the kept subspace (the bits of `mask` fixed to r's),
 // multiplied in FP64.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_rescale(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, double scale) {
-    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= groups) return;
-    const uint64_t i = deposit(t, dep);
-    const cplx<R> p = psi[i];
-    psi[i] = cplx<R>{(R)((double)p.re * scale), (R)((double)p.im * scale)};
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < groups; t += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t i = deposit(t, dep);
+        const cplx<R> p = psi[i];
+        psi[i] = cplx<R>{(R)((double)p.re * scale), (R)((double)p.im * scale)};
+    }
 }
 
 // get_probabilities (quant.rs:434-454): p_i = |psi_i|^2 * inv.
@@ -174,6 +182,44 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_probs(const cplx<R>* __restrict__ psi, uint64_t n, double inv, double* __restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) out[i] = weight(psi[i]) * inv;
+}
+
+// Comparison of two registers in FP64: per block, <a|b> (re, im), |a|^2,
+// |b|^2 and max |a_i - b_i| over 4096 amplitudes; the host folds the
+// partials in block order.
+template <class RA, class RB>
+__global__ __launch_bounds__(kBlock) void k_compare(const cplx<RA>* __restrict__ a, const cplx<RB>* __restrict__ b,
+                                                    uint64_t len, double* __restrict__ part) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint64_t base = (uint64_t)blockIdx.x * 4096u;
+    for (uint32_t k = threadIdx.x; k < 4096u; k += kBlock) {
+        const uint64_t i = base + k;
+        if (i >= len) break;
+        const double ar = (double)a[i].re, ai = (double)a[i].im, br = (double)b[i].re, bi = (double)b[i].im;
+        s[0] += ar * br + ai * bi;   // <a|b> = conj(a) b
+        s[1] += ar * bi - ai * br;
+        s[2] += ar * ar + ai * ai;
+        s[3] += br * br + bi * bi;
+        const double dr = ar - br, di = ai - bi;
+        s[4] = fmax(s[4], sqrt(dr * dr + di * di));
+    }
+    __shared__ double sh[5][kBlock];
+    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (unsigned h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + h];
+            sh[4][threadIdx.x] = fmax(sh[4][threadIdx.x], sh[4][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[(uint64_t)blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <class RA, class RB>
+void launch_compare(const void* a, const void* b, uint64_t len, double* part, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_compare<RA, RB>), dim3(blocks), dim3(kBlock), 0, s, static_cast<const cplx<RA>*>(a),
+                       static_cast<const cplx<RB>*>(b), len, part);
 }
 
 unsigned log2u(uint64_t x) { return 63u - (unsigned)__builtin_clzll(x); }
@@ -222,8 +268,9 @@ int build_levels(qvnt_reg* r, uint64_t count, const TreeInfo& info) {
         QVNT_HIP(hipGetLastError());
         return QVNT_OK;
     }
-    hipLaunchKernelGGL(k_leaf<R>, dim3((unsigned)(count >> kLeafLog)), dim3(kBlock), 0, r->stream, psi,
-                       r->tree + info.off[kLeafLog]);
+    const uint64_t nleaves = count >> kLeafLog;
+    hipLaunchKernelGGL(k_leaf<R>, dim3((unsigned)(nleaves < kMaxGrid ? nleaves : kMaxGrid)), dim3(kBlock), 0, r->stream, psi,
+                       nleaves, r->tree + info.off[kLeafLog]);
     QVNT_HIP(hipGetLastError());
     for (uint32_t L = kLeafLog; L < info.top;) {
         const uint32_t steps = (info.top - L) < kLeafLog ? (info.top - L) : kLeafLog;
@@ -281,7 +328,8 @@ int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint
 
 int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm) {
     {
-        const dim3 grid((unsigned)((r->len + kBlock - 1) / kBlock));
+        const uint64_t nb = (r->len + kBlock - 1) / kBlock;
+        const dim3 grid((unsigned)(nb < kMaxGrid ? nb : kMaxGrid));
         Timed t(r, QVNT_K_COLLAPSE, 2.0 * (double)r->amp_bytes() * (double)r->len);
         if (r->prec == QVNT_F64)
             hipLaunchKernelGGL(k_collapse<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), r->len, outcome, mask);
@@ -304,7 +352,8 @@ int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm) {
     }
     d.set = outcome & mask;
     const uint64_t groups = r->len >> d.n;
-    const dim3 grid((unsigned)((groups + kBlock - 1) / kBlock));
+    const uint64_t nb = (groups + kBlock - 1) / kBlock;
+    const dim3 grid((unsigned)(nb < kMaxGrid ? nb : kMaxGrid));
     Timed t(r, QVNT_K_COLLAPSE, 2.0 * (double)r->amp_bytes() * (double)groups);
     if (r->prec == QVNT_F64)
         hipLaunchKernelGGL(k_rescale<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), groups, d, scale);
@@ -334,6 +383,31 @@ int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n) {
     }
     (void)hipFree(buf);
     return rc;
+}
+
+int compare(qvnt_reg* a, qvnt_reg* b, double* out) {
+    const uint64_t len = a->len;
+    const unsigned blocks = (unsigned)((len + 4095) / 4096);
+    double* part = nullptr;
+    QVNT_HIP(hipStreamSynchronize(b->stream));
+    if (hipMalloc(&part, (size_t)blocks * 5 * sizeof(double)) != hipSuccess) return fail(QVNT_E_OOM, "comparison partials");
+    if (a->prec == QVNT_F64 && b->prec == QVNT_F64) launch_compare<double, double>(a->psi, b->psi, len, part, blocks, a->stream);
+    else if (a->prec == QVNT_F64) launch_compare<double, float>(a->psi, b->psi, len, part, blocks, a->stream);
+    else if (b->prec == QVNT_F64) launch_compare<float, double>(a->psi, b->psi, len, part, blocks, a->stream);
+    else launch_compare<float, float>(a->psi, b->psi, len, part, blocks, a->stream);
+    std::vector<double> host((size_t)blocks * 5);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host.data(), part, host.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return fail(QVNT_E_HIP, std::string("compare: ") + hipGetErrorString(e));
+    double s[5] = {0, 0, 0, 0, 0};
+    for (unsigned k = 0; k < blocks; ++k) {
+        for (int q = 0; q < 4; ++q) s[q] += host[(size_t)k * 5 + q];
+        s[4] = std::max(s[4], host[(size_t)k * 5 + 4]);
+    }
+    for (int q = 0; q < 5; ++q) out[q] = s[q];
+    return QVNT_OK;
 }
 
 }  // namespace qvnt

@@ -27,9 +27,16 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// A dispatch carries its size in work-items as a 32-bit number, so grids
+// are capped at kMaxGrid blocks and every kernel strides over chunks of
+// kBlock * U orbits.
+constexpr uint64_t kMaxGrid = 1u << 20;
+
 template <int K, class R, int U>
 __global__ __launch_bounds__(kBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
     cplx<R> v[U];
     uint64_t idx[U];
 #pragma unroll
@@ -43,11 +50,14 @@ __global__ __launch_bounds__(kBlock) void k_diag(cplx<R>* __restrict__ psi, uint
         const uint64_t t = base + (uint64_t)u * kBlock;
         if (t < groups) psi[idx[u]] = f_diag<K>(g, idx[u], v[u]);
     }
+  }
 }
 
 template <int K, class R, int U>
 __global__ __launch_bounds__(kBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
     cplx<R> v0[U], v1[U];
     uint64_t i0[U];
 #pragma unroll
@@ -70,11 +80,14 @@ __global__ __launch_bounds__(kBlock) void k_pair(cplx<R>* __restrict__ psi, uint
             psi[i1] = o1;
         }
     }
+  }
 }
 
 template <int K, class R, int U>
 __global__ __launch_bounds__(kBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
     cplx<R> v[U][4];
     uint64_t i0[U];
 #pragma unroll
@@ -101,13 +114,14 @@ __global__ __launch_bounds__(kBlock) void k_quad(cplx<R>* __restrict__ psi, uint
             psi[i0[u] | g.a | g.b] = o[3];
         }
     }
+  }
 }
 
 // QReg::with_state (quant.rs:129-150): zeros, psi[state] = 1.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_init(cplx<R>* __restrict__ psi, uint64_t len, uint64_t state) {
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < len) psi[i] = cplx<R>{i == state ? (R)1 : (R)0, (R)0};
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += (uint64_t)gridDim.x * kBlock)
+        psi[i] = cplx<R>{i == state ? (R)1 : (R)0, (R)0};
 }
 
 // ----------------------------------------------------------- host side
@@ -150,7 +164,8 @@ template <class R, int U>
 int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
     const Par<R> p = make_par<R>(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
-    const dim3 grid((unsigned)((groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U)));
+    const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+    const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid));
     const dim3 block(kBlock);
     hipStream_t s = r->stream;
 #define QVNT_CASE(KIND, KERNEL)                                            \
@@ -282,7 +297,8 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
 }
 
 int launch_init(qvnt_reg* r, uint64_t state) {
-    const dim3 grid((unsigned)((r->len + kBlock - 1) / kBlock));
+    const uint64_t nb = (r->len + kBlock - 1) / kBlock;
+    const dim3 grid((unsigned)(nb < kMaxGrid ? nb : kMaxGrid));
     Timed t(r, QVNT_K_OTHER, (double)r->amp_bytes() * (double)r->len);
     if (r->prec == QVNT_F64)
         hipLaunchKernelGGL(k_init<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), r->len, state);

@@ -219,6 +219,14 @@ int qvnt_reg_measure(qvnt_reg* r, double u, uint32_t flags, uint64_t* out) {
     return qvnt_reg_measure_mask(r, (1ull << r->q_num) - 1u, u, flags, out);
 }
 
+int qvnt_reg_compare(qvnt_reg* a, qvnt_reg* b, double out[5]) {
+    if (int rc = check_reg(a)) return rc;
+    if (int rc = check_reg(b)) return rc;
+    if (!out) return fail(QVNT_E_ARG, "null output");
+    if (a->len != b->len) return fail(QVNT_E_ARG, "registers differ in size");
+    return compare(a, b, out);
+}
+
 int qvnt_reg_get_state(qvnt_reg* r, void* dst, size_t bytes) {
     if (int rc = check_reg(r)) return rc;
     if (!dst || bytes != r->len * r->amp_bytes()) return fail(QVNT_E_ARG, "state copy needs len * amplitude bytes");

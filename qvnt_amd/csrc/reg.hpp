@@ -71,6 +71,7 @@ int tree_build(qvnt_reg* r, uint64_t count, double* total_out);
 int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out);
 int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm);
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n);
+int compare(qvnt_reg* a, qvnt_reg* b, double* out);
 
 // fused.hip
 int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes);

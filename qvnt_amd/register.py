@@ -146,6 +146,14 @@ class QReg:
                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return out
 
+    def compare(self, other):
+        """(max |a_i - b_i|, infidelity, <a|b>, |a|^2, |b|^2), computed on the device in FP64."""
+        out = (ctypes.c_double * 5)()
+        check(lib.qvnt_reg_compare(self._h, other.handle, out))
+        ov = complex(out[0], out[1])
+        infid = 1.0 - abs(ov) ** 2 / (out[2] * out[3]) if out[2] > 0 and out[3] > 0 else float("nan")
+        return out[4], infid, ov, out[2], out[3]
+
     def get_state(self):
         dt = np.complex128 if self.prec == "f64" else np.complex64
         out = np.empty(len(self), dt)

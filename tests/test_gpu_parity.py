@@ -280,3 +280,25 @@ def test_large_register_round_trip(n):
         draws = [qvnt_amd.uniform(4, n, k) for k in range(16)]
         outs = r.sample_mask((1 << n) - 1, draws)
         assert all(int(o) == start for o in outs)
+
+
+def test_compare_two_registers():
+    """qvnt_reg_compare against numpy on the copied-out states (FP64 vs FP32)."""
+    n = 12
+    g = np.random.default_rng(99)
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    specs = random_circuit(g, n, 60)
+    with QReg(n) as a, QReg(n, prec="f32") as b:
+        a.set_state(psi)
+        b.set_state(psi.astype(np.complex64))
+        a.apply(build(specs, op))
+        b.apply(build(specs, op))
+        err, infid, ov, na, nb = a.compare(b)
+        sa, sb = a.get_state(), b.get_state().astype(np.complex128)
+    assert abs(err - np.max(np.abs(sa - sb))) <= 1e-12
+    assert abs(ov - np.vdot(sa, sb)) <= 1e-12
+    assert abs(na - np.vdot(sa, sa).real) <= 1e-12 and abs(nb - np.vdot(sb, sb).real) <= 1e-12
+    want = 1.0 - abs(np.vdot(sa, sb)) ** 2 / (np.vdot(sa, sa).real * np.vdot(sb, sb).real)
+    assert abs(infid - want) <= 1e-12
+    assert 0.0 <= infid < 1e-5
