@@ -188,6 +188,94 @@ def tfim_pair(args):
                       "steps": rows}), flush=True)
 
 
+def serve(args):
+    """Serving workload: --requests circuits (n log-uniform in [12, 30],
+    100-2000 mixed gates, 1024 seeded shots of measure_mask on 8 random
+    qubits), three ways:
+      serial_plain  one request at a time, plain form (a closed loop: the next
+                    request is submitted when the last has finished);
+      serial_fused  one request at a time, fused form;
+      served        Poisson arrivals at --rate requests/s into the
+                    continuous-batching server (--slots streams, an HBM arena
+                    of --arena-gib), fused form.
+    Latency = completion - submission (submission includes validation and
+    the fused-program compile); circuits/s = requests / (last completion -
+    first submission)."""
+    import numpy as np
+    from qvnt_amd import circuits, op
+    from qvnt_amd.serve import Server
+    reqs = circuits.serving_requests(args.requests, n_lo=12, n_hi=args.qubits)
+    t0 = time.perf_counter()
+    ops = [circuits.build(specs, op) for _, specs, _, _ in reqs]
+    log("serve: %d requests built in %.1f s; qubits %s; gates %d..%d" % (
+        len(reqs), time.perf_counter() - t0, np.bincount([r[0] for r in reqs]).nonzero()[0].tolist(),
+        min(len(o) for o in ops), max(len(o) for o in ops)))
+    arena = int(args.arena_gib * 2**30)
+
+    def stats(res, label):
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        bad = [r["id"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError("%s: requests %s failed" % (label, bad[:8]))
+        out = {"circuits_per_s": round(len(res) / span, 3), "seconds": round(span, 3),
+               "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+               "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+               "mean_ms": round(1e3 * float(lat.mean()), 2)}
+        log("%s: %.2f circuits/s, p50 %.1f ms, p99 %.1f ms (%.1f s)" % (label, out["circuits_per_s"], out["p50_ms"],
+                                                                        out["p99_ms"], span))
+        return out
+
+    def closed_loop(fused, slots=1):
+        res = []
+        with Server(arena, slots=slots) as srv:
+            for k, (n, _, mask, draws) in enumerate(reqs):
+                srv.submit(k, n, ops[k], mask, draws, fused=fused)
+                res += srv.drain(1)
+        return res
+
+    out = {}
+    totals = {}
+    if not args.no_plain:
+        res = closed_loop(False)
+        out["serial_plain"] = stats(res, "serial plain")
+        totals["plain"] = {r["id"]: r["total"] for r in res}
+    res = closed_loop(True)
+    out["serial_fused"] = stats(res, "serial fused")
+    totals["fused"] = {r["id"]: r["total"] for r in res}
+    serial_outcomes = {r["id"]: r["outcomes"] for r in res}
+    # open loop: Poisson arrivals at the offered rate
+    gaps = np.random.default_rng(circuits.SEED).exponential(1.0 / args.rate, len(reqs))
+    with Server(arena, slots=args.slots) as srv:
+        start = srv.now()
+        due = start + np.cumsum(gaps) - gaps[0]
+        for k, (n, _, mask, draws) in enumerate(reqs):
+            wait = due[k] - srv.now()
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(k, n, ops[k], mask, draws, fused=True)
+        res = srv.drain(len(reqs))
+        _, _, peak = srv.pending()
+    out["served"] = stats(res, "served (rate %.1f/s, %d slots)" % (args.rate, args.slots))
+    out["served"]["offered_rate"] = args.rate
+    out["served"]["slots"] = args.slots
+    out["served"]["arena_peak_gib"] = round(peak / 2**30, 2)
+    same = all(np.array_equal(r["outcomes"], serial_outcomes[r["id"]]) and r["total"] == totals["fused"][r["id"]]
+               for r in res)
+    out["served"]["outcomes_equal_serial_fused"] = bool(same)
+    if "plain" in totals:
+        out["max_total_diff_fused_vs_plain"] = max(abs(totals["fused"][k] - totals["plain"][k]) for k in totals["plain"])
+    base = out.get("serial_plain", out["serial_fused"])["circuits_per_s"]
+    line = {"metric": METRIC, "value": out["served"]["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
+            "steps": len(reqs), "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded serving requests)",
+            "config": {"workload": "serving-%d-requests-12..%dq-100..2000-gates-1024-shots" % (len(reqs), args.qubits),
+                       "arena_gib": args.arena_gib},
+            "speedup_vs_serial_plain": round(out["served"]["circuits_per_s"] / base, 2) if base else None}
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -208,7 +296,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair"])
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve"])
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
+    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent requests on the device")
+    ap.add_argument("--arena-gib", type=float, default=160.0, help="serve: HBM admission budget")
     ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
     ap.add_argument("--fused-steps", type=int, default=50, help="tfim: Trotter steps through the fused form")
     args = ap.parse_args()
@@ -224,6 +316,8 @@ def main():
         return tfim(args)
     if args.workload == "tfim-pair":
         return tfim_pair(args)
+    if args.workload == "serve":
+        return serve(args)
 
     n = args.qubits
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]

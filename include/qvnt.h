@@ -222,6 +222,48 @@ int qvnt_reg_set_state(qvnt_reg* reg, const void* src, size_t bytes);
 /* Wait for all work queued on the register's stream. */
 int qvnt_reg_sync(qvnt_reg* reg);
 
+/* ----------------------------------------------------------- serving
+ * A continuous-batching request server (no reference counterpart).  A
+ * request is one circuit on a fresh register of q_num qubits (basis state
+ * init_state) followed by `shots` non-destructive draws of measure_mask on
+ * the final state (each equals measure_mask on a copy of the register).
+ * Requests are admitted as stream slots and space in the server's HBM arena
+ * free up; requests in different slots overlap on the device; results do
+ * not depend on what runs beside them. */
+typedef struct qvnt_server qvnt_server;
+typedef struct qvnt_request {
+    uint64_t id;
+    uint32_t q_num;
+    uint32_t prec;            /* qvnt_prec                                     */
+    uint64_t init_state;
+    const qvnt_op* op;        /* copied at submit                              */
+    uint32_t fused;           /* 1: fused form (qvnt_reg_apply_fused); 0: plain */
+    uint32_t max_qubits;      /* fused tile qubits, 0 = default                */
+    uint64_t mask;            /* measured qubits                               */
+    uint64_t shots;           /* 1 .. 2^20                                     */
+    const double* draws;      /* shots uniform draws in [0,1), copied          */
+    uint64_t* outcomes;       /* caller-owned; filled (r_k & mask) before the
+                                 result is returned by qvnt_server_poll       */
+} qvnt_request;
+typedef struct qvnt_result {
+    uint64_t id;
+    int32_t status;           /* QVNT_OK, or QVNT_E_OOM if the request exceeds
+                                 the arena, or another error code             */
+    uint32_t passes;          /* kernel passes over the state                  */
+    double total;             /* |psi|^2 of the final state                    */
+    double t_submit, t_admit, t_done;  /* seconds on the server clock         */
+} qvnt_result;
+/* hbm_bytes: the arena (admission budget); slots: concurrent requests. */
+int qvnt_server_new(uint64_t hbm_bytes, uint32_t slots, qvnt_server** out);
+int qvnt_server_submit(qvnt_server* s, const qvnt_request* req);
+/* Up to cap finished results; waits up to wait_ms for the first one. */
+int qvnt_server_poll(qvnt_server* s, qvnt_result* out, uint32_t cap, uint32_t* n, int wait_ms);
+int qvnt_server_pending(qvnt_server* s, uint32_t* queued, uint32_t* running, uint64_t* peak_bytes);
+/* Seconds since the server started (the clock of t_submit/t_admit/t_done). */
+double qvnt_server_now(qvnt_server* s);
+/* Finishes in-flight requests, drops queued ones, frees everything. */
+int qvnt_server_free(qvnt_server* s);
+
 /* --------------------------------------------------------- instrumentation
  * Per-kernel-class device timing with HIP events recorded on the register's
  * stream around every launch (off by default). */

@@ -13,6 +13,11 @@ Workloads:
                 even/odd nearest-neighbour pairs, one controlled x.
   tfim          transverse-field Ising Trotter steps on a ring: rzz(2 J dt)
                 on every ring edge, then rx(2 h_i dt) on every qubit.
+  random_mixed  gates drawn uniformly from {h, x, y, z, s, t, rx, ry, rz, u3,
+                rxx, ryy, rzz, swap, iswap, controlled x}, angles U[0, 2pi):
+                the batched-register and serving workloads.
+  serving       requests with n log-uniform in [12, 30], 100-2000
+                random_mixed gates, shots on 8 random qubits.
 """
 import math
 
@@ -67,6 +72,44 @@ def tfim_step(n, dt=0.05, J=1.0, seed=SEED):
     return specs
 
 
+MIXED = ("h", "x", "y", "z", "s", "t", "rx", "ry", "rz", "u3", "rxx", "ryy", "rzz", "swap", "i_swap", "cx")
+
+
+def random_mixed(n, count, g):
+    """`count` gates on n >= 2 qubits from the mixed set, drawn from generator g."""
+    specs = []
+    for _ in range(count):
+        ctor = MIXED[int(g.integers(len(MIXED)))]
+        a, b = (int(v) for v in g.choice(n, 2, replace=False))
+        if ctor in ("h", "x", "y", "z", "s", "t"):
+            specs.append((ctor, (1 << a,), 0))
+        elif ctor in ("rx", "ry", "rz"):
+            specs.append((ctor, (float(g.uniform(0.0, TWO_PI)), 1 << a), 0))
+        elif ctor == "u3":
+            specs.append(("u3", tuple(float(v) for v in g.uniform(0.0, TWO_PI, 3)) + (1 << a,), 0))
+        elif ctor in ("rxx", "ryy", "rzz"):
+            specs.append((ctor, (float(g.uniform(0.0, TWO_PI)), (1 << a) | (1 << b)), 0))
+        elif ctor in ("swap", "i_swap"):
+            specs.append((ctor, ((1 << a) | (1 << b),), 0))
+        else:
+            specs.append(("x", (1 << a,), 1 << b))
+    return specs
+
+
+def serving_requests(count, seed=SEED, n_lo=12, n_hi=30, g_lo=100, g_hi=2000, shots=1024):
+    """Requests of the serving workload: (n, specs, mask, draws)."""
+    g = rng(seed, 5)
+    out = []
+    for k in range(count):
+        n = int(round(math.exp(g.uniform(math.log(n_lo), math.log(n_hi)))))
+        specs = random_mixed(n, int(g.integers(g_lo, g_hi + 1)), g)
+        qs = g.choice(n, min(8, n), replace=False)
+        mask = sum(1 << int(q) for q in qs)
+        draws = rng(seed, 100000 + k).random(shots)
+        out.append((n, specs, mask, draws))
+    return out
+
+
 def build(specs, ns):
     """One MultiOp (queue order) from specs, with namespace `ns` (op::)."""
     out = ns.id()
@@ -76,7 +119,7 @@ def build(specs, ns):
             o = o.c(ctrl)
             if o is None:
                 raise ValueError("control overlaps the gate in spec %r" % ((ctor, args, ctrl),))
-        out = out * o
+        out *= o
     return out
 
 

@@ -200,30 +200,36 @@ __global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass 
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Uploads the program (stream-ordered) and launches one kernel per pass.
-int run_program(qvnt_reg* r, const Program& prog) {
-    static thread_local std::vector<unsigned char> host;
-    const size_t ph_off = 0;
-    const size_t op_off = align256(ph_off + prog.phases.size() * sizeof(FPhase));
-    const size_t cf_off = align256(op_off + prog.ops.size() * sizeof(FOp));
-    const size_t total = cf_off + prog.coef.size() * sizeof(double) + 256;
-    if (total > r->prog_cap) {
-        QVNT_HIP(hipStreamSynchronize(r->stream));
-        if (r->prog) QVNT_HIP(hipFree(r->prog));
-        r->prog = nullptr;
-        r->prog_cap = 0;
-        if (hipMalloc(&r->prog, total) != hipSuccess) return fail(QVNT_E_OOM, "device memory for the fused program");
-        r->prog_cap = total;
-    }
-    host.assign(total, 0);
-    if (!prog.phases.empty()) std::memcpy(host.data() + ph_off, prog.phases.data(), prog.phases.size() * sizeof(FPhase));
-    if (!prog.ops.empty()) std::memcpy(host.data() + op_off, prog.ops.data(), prog.ops.size() * sizeof(FOp));
-    if (!prog.coef.empty()) std::memcpy(host.data() + cf_off, prog.coef.data(), prog.coef.size() * sizeof(double));
-    QVNT_HIP(hipMemcpyAsync(r->prog, host.data(), total, hipMemcpyHostToDevice, r->stream));
-    unsigned char* d = static_cast<unsigned char*>(r->prog);
-    const FPhase* dph = reinterpret_cast<const FPhase*>(d + ph_off);
-    const FOp* dop = reinterpret_cast<const FOp*>(d + op_off);
-    const double* dcf = reinterpret_cast<const double*>(d + cf_off);
+struct Layout {
+    size_t ph_off, op_off, cf_off, total;
+};
+Layout layout_of(const Program& prog) {
+    Layout L;
+    L.ph_off = 0;
+    L.op_off = align256(L.ph_off + prog.phases.size() * sizeof(FPhase));
+    L.cf_off = align256(L.op_off + prog.ops.size() * sizeof(FOp));
+    L.total = L.cf_off + prog.coef.size() * sizeof(double) + 256;
+    return L;
+}
+
+}  // namespace
+
+size_t program_bytes(const Program& prog) { return layout_of(prog).total; }
+
+void pack_program(const Program& prog, unsigned char* host) {
+    const Layout L = layout_of(prog);
+    std::memset(host, 0, L.total);
+    if (!prog.phases.empty()) std::memcpy(host + L.ph_off, prog.phases.data(), prog.phases.size() * sizeof(FPhase));
+    if (!prog.ops.empty()) std::memcpy(host + L.op_off, prog.ops.data(), prog.ops.size() * sizeof(FOp));
+    if (!prog.coef.empty()) std::memcpy(host + L.cf_off, prog.coef.data(), prog.coef.size() * sizeof(double));
+}
+
+int launch_program(qvnt_reg* r, const Program& prog, const void* dev) {
+    const Layout L = layout_of(prog);
+    const unsigned char* d = static_cast<const unsigned char*>(dev);
+    const FPhase* dph = reinterpret_cast<const FPhase*>(d + L.ph_off);
+    const FOp* dop = reinterpret_cast<const FOp*>(d + L.op_off);
+    const double* dcf = reinterpret_cast<const double*>(d + L.cf_off);
     for (const FPass& fp : prog.passes) {
         const uint64_t tiles = r->len >> fp.t;
         const size_t lds = ((size_t)1 << fp.t) * r->amp_bytes();
@@ -237,6 +243,27 @@ int run_program(qvnt_reg* r, const Program& prog) {
         QVNT_HIP(hipGetLastError());
     }
     return QVNT_OK;
+}
+
+namespace {
+
+// Uploads the program (stream-ordered, from pageable memory) and launches
+// one kernel per pass.
+int run_program(qvnt_reg* r, const Program& prog) {
+    static thread_local std::vector<unsigned char> host;
+    const size_t total = program_bytes(prog);
+    if (total > r->prog_cap) {
+        QVNT_HIP(hipStreamSynchronize(r->stream));
+        if (r->prog) QVNT_HIP(hipFree(r->prog));
+        r->prog = nullptr;
+        r->prog_cap = 0;
+        if (hipMalloc(&r->prog, total) != hipSuccess) return fail(QVNT_E_OOM, "device memory for the fused program");
+        r->prog_cap = total;
+    }
+    host.resize(total);
+    pack_program(prog, host.data());
+    QVNT_HIP(hipMemcpyAsync(r->prog, host.data(), total, hipMemcpyHostToDevice, r->stream));
+    return launch_program(r, prog, r->prog);
 }
 
 }  // namespace

@@ -301,25 +301,33 @@ int tree_build(qvnt_reg* r, uint64_t count, double* total_out) {
     return QVNT_OK;
 }
 
-int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out) {
+size_t tree_doubles(uint64_t count) {
     const TreeInfo info = tree_info(count);
-    int rc = ensure_workspace(r, (size_t)(info.off[info.top] + 1), shots);
+    return (size_t)(info.off[info.top] + 1);
+}
+
+int sample_launch(qvnt_reg* r, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev, double* total_dev) {
+    const TreeInfo info = tree_info(count);
+    if ((size_t)(info.off[info.top] + 1) > r->tree_cap) return fail(QVNT_E_ARG, "measurement tree workspace too small");
+    int rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
     if (rc) return rc;
-    rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
+    Timed t(r, QVNT_K_OTHER, 0.0);
+    if (r->prec == QVNT_F64)
+        hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                           static_cast<const c64*>(r->psi), (const double*)r->tree, info, u_dev, out_dev, total_dev);
+    else
+        hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                           static_cast<const c32*>(r->psi), (const double*)r->tree, info, u_dev, out_dev, total_dev);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out) {
+    int rc = ensure_workspace(r, tree_doubles(count), shots);
     if (rc) return rc;
     QVNT_HIP(hipMemcpyAsync(r->dscratch + 1, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
-    {
-        Timed t(r, QVNT_K_OTHER, 0.0);
-        if (r->prec == QVNT_F64)
-            hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                               static_cast<const c64*>(r->psi), (const double*)r->tree, info,
-                               (const double*)(r->dscratch + 1), r->iscratch, r->dscratch);
-        else
-            hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                               static_cast<const c32*>(r->psi), (const double*)r->tree, info,
-                               (const double*)(r->dscratch + 1), r->iscratch, r->dscratch);
-        QVNT_HIP(hipGetLastError());
-    }
+    rc = sample_launch(r, count, r->dscratch + 1, shots, r->iscratch, r->dscratch);
+    if (rc) return rc;
     QVNT_HIP(hipMemcpyAsync(out, r->iscratch, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
     QVNT_HIP(hipMemcpyAsync(total_out, r->dscratch, sizeof(double), hipMemcpyDeviceToHost, r->stream));
     QVNT_HIP(hipStreamSynchronize(r->stream));

@@ -114,12 +114,14 @@ int qvnt_reg_free(qvnt_reg* r) {
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
-    if (r->psi) (void)hipFree(r->psi);
-    if (r->tree) (void)hipFree(r->tree);
-    if (r->dscratch) (void)hipFree(r->dscratch);
-    if (r->iscratch) (void)hipFree(r->iscratch);
-    if (r->prog) (void)hipFree(r->prog);
-    if (r->stream) (void)hipStreamDestroy(r->stream);
+    if (r->owns_memory) {
+        if (r->psi) (void)hipFree(r->psi);
+        if (r->tree) (void)hipFree(r->tree);
+        if (r->dscratch) (void)hipFree(r->dscratch);
+        if (r->iscratch) (void)hipFree(r->iscratch);
+        if (r->prog) (void)hipFree(r->prog);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
+    }
     delete r;
     return QVNT_OK;
 }

@@ -37,6 +37,7 @@ struct qvnt_reg {
     size_t prog_cap = 0;
 
     // per-kernel-class timing
+    bool owns_memory = true;  // false: state, workspace and stream belong to a server slot
     bool timing = false;
     struct Pending { int kclass; hipEvent_t a, b; double bytes; };
     std::vector<Pending> pending;
@@ -69,11 +70,20 @@ int launch_init(qvnt_reg* r, uint64_t state);
 // measure.hip
 int tree_build(qvnt_reg* r, uint64_t count, double* total_out);
 int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out);
+// doubles of tree workspace the draws over `count` weights need
+size_t tree_doubles(uint64_t count);
+// enqueues the tree levels and one draw per u_dev[k] into out_dev (device
+// pointers; r->tree must hold tree_doubles(count)); no synchronisation
+int sample_launch(qvnt_reg* r, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev, double* total_dev);
 int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm);
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n);
 int compare(qvnt_reg* a, qvnt_reg* b, double* out);
 
 // fused.hip
 int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes);
+namespace fusion { struct Program; }
+size_t program_bytes(const fusion::Program& prog);
+void pack_program(const fusion::Program& prog, unsigned char* host);
+int launch_program(qvnt_reg* r, const fusion::Program& prog, const void* dev);
 
 }  // namespace qvnt

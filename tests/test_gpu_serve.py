@@ -1,0 +1,96 @@
+"""The continuous-batching server: a request's outcomes are the same alone
+and in the mix, and equal the oracle's for plain-form requests."""
+import numpy as np
+import pytest
+
+import qvnt_amd
+from qvnt_amd import QReg, circuits, op
+from qvnt_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def make_requests(count, seed, n_lo=6, n_hi=16, g_lo=20, g_hi=200, shots=256):
+    g = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        n = int(g.integers(n_lo, n_hi + 1))
+        specs = circuits.random_mixed(n, int(g.integers(g_lo, g_hi + 1)), g)
+        mask = int(sum(1 << int(q) for q in g.choice(n, min(8, n), replace=False)))
+        draws = np.array([qvnt_amd.uniform(seed, k, j) for j in range(shots)])
+        reqs.append((n, specs, mask, draws, bool(k % 3)))
+    return reqs
+
+
+def run(server, reqs, one_at_a_time):
+    results = {}
+    for k, (n, specs, mask, draws, fused) in enumerate(reqs):
+        server.submit(k, n, circuits.build(specs, op), mask, draws, fused=fused)
+        if one_at_a_time:
+            for r in server.drain(1):
+                results[r["id"]] = r
+    if not one_at_a_time:
+        for r in server.drain(len(reqs)):
+            results[r["id"]] = r
+    assert len(results) == len(reqs)
+    return results
+
+
+def test_alone_equals_mix(orc):
+    reqs = make_requests(48, 11)
+    with Server(4 << 30, slots=1) as s1:
+        alone = run(s1, reqs, True)
+    with Server(4 << 30, slots=8) as s8:
+        mix = run(s8, reqs, False)
+    for k, (n, specs, mask, draws, fused) in enumerate(reqs):
+        a, m = alone[k], mix[k]
+        assert a["status"] == 0 and m["status"] == 0
+        assert np.array_equal(a["outcomes"], m["outcomes"])
+        assert a["total"] == m["total"]
+        assert np.all((a["outcomes"] & ~np.uint64(mask)) == 0)
+        if not fused and n <= 14:
+            # plain requests are bit-exact with the oracle: same outcomes
+            ref = orc.Reg(n)
+            ref.apply(circuits.build(specs, orc.op))
+            psi = ref.state()
+            want = [orc.tree_sample(psi[: 1 << n], u) & mask for u in draws]
+            assert np.array_equal(a["outcomes"], np.array(want, np.uint64))
+
+
+def test_server_matches_register_path():
+    reqs = make_requests(12, 21, n_lo=10, n_hi=14)
+    with Server(1 << 30, slots=4) as s:
+        res = run(s, reqs, False)
+    for k, (n, specs, mask, draws, fused) in enumerate(reqs):
+        with QReg(n) as r:
+            if fused:
+                r.apply_fused(circuits.build(specs, op))
+            else:
+                r.apply(circuits.build(specs, op))
+            want = r.sample_mask(mask, draws)
+            assert r.get_absolute() == res[k]["total"]
+        assert np.array_equal(res[k]["outcomes"], want)
+
+
+def test_admission_by_budget():
+    """A request larger than the arena is refused; the rest are admitted as
+    space frees up (some wait) and the arena's peak stays within its size."""
+    budget = 40 << 20
+    g = np.random.default_rng(3)
+    small = circuits.build(circuits.random_mixed(18, 100, g), op)
+    body = small.clone()
+    for _ in range(3):
+        body *= body                             # 800 plain gates: each 20-qubit request runs for milliseconds
+    sizes = [20, 20, 19, 22, 18, 20, 19]         # 22 qubits (64 MiB) exceeds the arena; two 20-qubit states fill it
+    with Server(budget, slots=8) as s:
+        for k, n in enumerate(sizes):
+            s.submit(k, n, body, 0xF, np.full(16, 0.5), fused=False)
+        res = {r["id"]: r for r in s.drain(len(sizes))}
+        _, _, peak = s.pending()
+    assert res[3]["status"] == -3
+    assert all(res[k]["status"] == 0 for k in range(len(sizes)) if k != 3)
+    assert 0 < peak <= budget
+    waited = [k for k in res if k != 3 and res[k]["t_admit"] > res[k]["t_submit"] + 1e-3]
+    assert waited, "every request was admitted at once although the arena holds two 20-qubit states"
+    for k in res:
+        assert res[k]["t_submit"] <= res[k]["t_admit"] <= res[k]["t_done"]
