@@ -276,6 +276,85 @@ def serve(args):
     print(json.dumps(line), flush=True)
 
 
+def batch(args):
+    """Batched uneven registers: --registers registers (n uniform in [10, 20],
+    20-400 mixed gates, seeded), each from |0>.  The plain form applies every
+    register's circuit one kernel per SingleOp, register after register; the
+    batched form compiles all circuits (autotuned tile caps) and runs them
+    round by round, as one hipGraph, --steps times.  value = SingleOps/s of
+    the batched form over all registers."""
+    import numpy as np
+    from qvnt_amd import QReg, circuits, op
+    from qvnt_amd.batch import Batch
+    prec = args.prec
+    t0 = time.perf_counter()
+    regs = circuits.batched_registers(args.registers, n_lo=10, n_hi=args.qubits)
+    ops = [circuits.build(specs, op) for _, specs in regs]
+    single = sum(len(o) for o in ops)
+    amp = 16 if prec == "f64" else 8
+    state_gib = sum(amp << n for n, _ in regs) / 2**30
+    log("batch: %d registers, %d SingleOps, %.2f GiB of %s state (built in %.1f s)" % (
+        len(regs), single, state_gib, prec, time.perf_counter() - t0))
+    out = {}
+    sample = min(args.check_registers, len(regs))
+    plain_states = []
+    if not args.no_plain:
+        # every register, in chunks of 512 live registers (allocation outside the timed spans)
+        wall = 0.0
+        for c0 in range(0, len(regs), 512):
+            live = [QReg(n, prec=prec) for n, _ in regs[c0:c0 + 512]]
+            for r in live:
+                r.sync()
+            t0 = time.perf_counter()
+            for r, o in zip(live, ops[c0:c0 + 512]):
+                r.apply(o)
+            for r in live:
+                r.sync()
+            wall += time.perf_counter() - t0
+            plain_states += [live[k - c0].get_state()[: 1 << regs[k][0]] for k in range(c0, min(sample, c0 + 512))]
+            for r in live:
+                r.close()
+        out["plain"] = {"value": round(single / wall, 1), "seconds": round(wall, 3)}
+        log("plain: %.2f s -> %.0f SingleOps/s" % (wall, single / wall))
+    with Batch([n for n, _ in regs], prec=prec) as b:
+        t0 = time.perf_counter()
+        info = b.compile(ops, max_qubits=args.tile_cap)
+        comp = time.perf_counter() - t0
+        log("compile%s: %.2f s, %d rounds, %d launches, %d passes; caps %s" % (
+            " + autotune" if not args.tile_cap else "", comp, info["rounds"], info["launches"], info["passes"],
+            info["tile_cap"]))
+        b.run(graph=True)
+        b.sync()
+        errs = []
+        for k in range(sample):
+            got = b.get_state(k)
+            if plain_states:
+                errs.append(float(np.max(np.abs(got.astype(np.complex128) - plain_states[k]))))
+        for _ in range(args.warmup):
+            b.run(graph=True)
+        ms_graph = b.time(args.steps, graph=True)
+        ms_eager = b.time(max(1, args.steps // 2), graph=False)
+    gbs = info["bytes"] / (ms_graph * 1e-3) / 1e9
+    value = single / (ms_graph * 1e-3)
+    log("batched: %.2f ms/run (graph), %.2f ms/run (eager) -> %.0f SingleOps/s, %.0f GB/s algorithmic" % (
+        ms_graph, ms_eager, value, gbs))
+    out["batched"] = {"value": round(value, 1), "ms_per_run_graph": round(ms_graph, 3),
+                      "ms_per_run_eager": round(ms_eager, 3), "compile_s": round(comp, 2),
+                      "rounds": info["rounds"], "launches": info["launches"], "passes": info["passes"],
+                      "tiles": info["tiles"], "tile_cap": info["tile_cap"], "tune_ms": info["tune_ms"],
+                      "max_abs_err_vs_plain": max(errs) if errs else None, "checked_registers": len(errs)}
+    line = {"metric": METRIC, "value": round(value, 1), "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": prec, "data": "synthetic (seeded batched registers)",
+            "config": {"workload": "batched-%d-registers-10..%dq-20..400-gates-%s" % (len(regs), args.qubits, prec),
+                       "registers": len(regs), "single_ops": single, "state_gib": round(state_gib, 2)},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch"},
+            "speedup_vs_plain": round(value / out["plain"]["value"], 2) if "plain" in out else None}
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -291,12 +370,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--qubits", type=int, default=None,
+                    help="register size (largest size for serve/batch); default per workload")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve"])
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch"])
+    ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
+    ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
+    ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
     ap.add_argument("--slots", type=int, default=8, help="serve: concurrent requests on the device")
@@ -309,6 +392,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.qubits is None:
+        args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
+                       "batch": 20}[args.workload]
+
     import qvnt_amd
     from qvnt_amd import QReg, circuits, op
 
@@ -318,6 +405,8 @@ def main():
         return tfim_pair(args)
     if args.workload == "serve":
         return serve(args)
+    if args.workload == "batch":
+        return batch(args)
 
     n = args.qubits
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]

@@ -287,6 +287,49 @@ int qvnt_reg_set_timing(qvnt_reg* reg, int enable);
  * QVNT_KCLASS_COUNT entries into out, and (if reset) clears them. */
 int qvnt_reg_kernel_stats(qvnt_reg* reg, qvnt_kstats* out, int reset);
 
+/* ------------------------------------------------------------------ */
+/* Batched uneven registers (no counterpart in the reference, which runs
+ * one register at a time): many independent registers of 6..30 qubits in
+ * one device arena, each with its own circuit, compiled into fused passes
+ * and run round by round (pass k of every register in one launch per tile
+ * size), optionally as one captured hipGraph. */
+typedef struct qvnt_batch qvnt_batch;
+typedef struct {
+    uint32_t rounds;          /* sequential rounds: the most passes of any register */
+    uint32_t launches;        /* kernel launches per run */
+    uint64_t passes;          /* register passes per run */
+    uint64_t tiles;           /* tiles (workgroups) per run */
+    uint64_t single_ops;      /* reference SingleOps of all circuits */
+    double bytes;             /* algorithmic HBM bytes per run: 2 * amp * 2^n per pass */
+    uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
+    double tune_ms[32][5];    /* autotuner: ms per run of the registers of each qubit count
+                                 with caps 8, 9, 10, 11, 12 (0: not timed / same as a smaller cap) */
+} qvnt_batch_info;
+/* Registers k = 0..count-1 of q_nums[k] qubits, each set to basis state
+ * states[k] (null: all 0). */
+int qvnt_batch_new(uint32_t count, const uint32_t* q_nums, const uint64_t* states, qvnt_prec prec, qvnt_batch** out);
+uint32_t qvnt_batch_count(const qvnt_batch* b);
+/* Every register back to its initial basis state (stream-ordered). */
+int qvnt_batch_reset(qvnt_batch* b);
+/* Compiles ops[k] (queue order, as qvnt_reg_apply) for register k.
+ * max_qubits > 0 caps every tile at that many qubits (6..12); 0 runs the
+ * autotuner, which times every cap on the registers of each size and
+ * leaves the registers reset to their initial states. */
+int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, qvnt_batch_info* info);
+/* Queues one application of every compiled circuit (graph != 0: as one
+ * hipGraph, captured at the first such call after a compile). */
+int qvnt_batch_run(qvnt_batch* b, int graph);
+/* Runs `runs` times between two device events; *ms = milliseconds per run. */
+int qvnt_batch_time(qvnt_batch* b, uint32_t runs, int graph, double* ms);
+int qvnt_batch_sync(qvnt_batch* b);
+/* Register k's amplitudes (2^q interleaved complex of the batch precision). */
+int qvnt_batch_get_state(qvnt_batch* b, uint32_t k, void* dst, size_t bytes);
+/* Shots of measure_mask on register k without collapse (as
+ * qvnt_reg_sample_mask); *total (may be null) = sum |psi|^2. */
+int qvnt_batch_sample_mask(qvnt_batch* b, uint32_t k, uint64_t mask, const double* u, size_t shots, uint64_t* out,
+                           double* total);
+int qvnt_batch_free(qvnt_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

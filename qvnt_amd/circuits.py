@@ -77,18 +77,21 @@ MIXED = ("h", "x", "y", "z", "s", "t", "rx", "ry", "rz", "u3", "rxx", "ryy", "rz
 
 def random_mixed(n, count, g):
     """`count` gates on n >= 2 qubits from the mixed set, drawn from generator g."""
+    ctors = g.integers(len(MIXED), size=count)
+    qa = g.integers(n, size=count)
+    qb = (qa + 1 + g.integers(n - 1, size=count)) % n      # a second, distinct qubit
+    ang = g.uniform(0.0, TWO_PI, size=(count, 3))
     specs = []
-    for _ in range(count):
-        ctor = MIXED[int(g.integers(len(MIXED)))]
-        a, b = (int(v) for v in g.choice(n, 2, replace=False))
+    for ctor, a, b, (t0, t1, t2) in zip(ctors.tolist(), qa.tolist(), qb.tolist(), ang.tolist()):
+        ctor = MIXED[ctor]
         if ctor in ("h", "x", "y", "z", "s", "t"):
             specs.append((ctor, (1 << a,), 0))
         elif ctor in ("rx", "ry", "rz"):
-            specs.append((ctor, (float(g.uniform(0.0, TWO_PI)), 1 << a), 0))
+            specs.append((ctor, (t0, 1 << a), 0))
         elif ctor == "u3":
-            specs.append(("u3", tuple(float(v) for v in g.uniform(0.0, TWO_PI, 3)) + (1 << a,), 0))
+            specs.append(("u3", (t0, t1, t2, 1 << a), 0))
         elif ctor in ("rxx", "ryy", "rzz"):
-            specs.append((ctor, (float(g.uniform(0.0, TWO_PI)), (1 << a) | (1 << b)), 0))
+            specs.append((ctor, (t0, (1 << a) | (1 << b)), 0))
         elif ctor in ("swap", "i_swap"):
             specs.append((ctor, ((1 << a) | (1 << b),), 0))
         else:
@@ -140,3 +143,14 @@ def single_ops(specs):
         else:
             total += 1
     return total
+
+
+def batched_registers(count, seed=SEED, n_lo=10, n_hi=20, g_lo=20, g_hi=400):
+    """The batched-register workload: (n, specs) per register, n uniform in
+    [n_lo, n_hi], g_lo..g_hi random_mixed gates."""
+    g = rng(seed, 4)
+    out = []
+    for _ in range(count):
+        n = int(g.integers(n_lo, n_hi + 1))
+        out.append((n, random_mixed(n, int(g.integers(g_lo, g_hi + 1)), g)))
+    return out

@@ -137,15 +137,17 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
     }
 }
 
+// One tile of one pass: read 2^t amplitudes, run the pass's phases, write
+// them back.  `tile_index` enumerates the tiles (the bits outside the
+// tile's qubit set); the pass and program live in scalar-loadable memory.
 template <class R>
-__global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
-                                               const FOp* __restrict__ ops, const double* __restrict__ coef) {
-    extern __shared__ unsigned char smem[];
-    cplx<R>* lds = reinterpret_cast<cplx<R>*>(smem);
+__device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPass& pass, const FPhase* __restrict__ phases,
+                                           const FOp* __restrict__ ops, const double* __restrict__ coef,
+                                           uint64_t tile_index, cplx<R>* lds) {
     const uint32_t tid = threadIdx.x, nth = pass.nthreads;
 
     // global index of local 0: insert zeros at the tile's bit positions
-    uint64_t tile = blockIdx.x;
+    uint64_t tile = tile_index;
     for (uint32_t k = 0; k < pass.t; ++k) {
         const uint64_t lo = tile & ((1ull << pass.tpos[k]) - 1u);
         tile = ((tile ^ lo) << 1) | lo;
@@ -198,6 +200,28 @@ __global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass 
     for (int j = 0; j < kRegs; ++j) psi[base + pass.hoff[j]] = v[j];
 }
 
+template <class R>
+__global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
+                                               const FOp* __restrict__ ops, const double* __restrict__ coef) {
+    extern __shared__ unsigned char smem[];
+    fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem));
+}
+
+// The batched form: the tiles of one pass of many registers in one launch
+// (every item's pass has the launch's t).  Block b runs tile
+// b - items[map[b]].block0 of item map[b]; registers, passes and programs
+// are addressed by offsets from the two base pointers.
+template <class R>
+__global__ __launch_bounds__(256) void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
+                                               const BItem* __restrict__ items, const uint32_t* __restrict__ map) {
+    extern __shared__ unsigned char smem[];
+    const BItem it = items[map[blockIdx.x]];
+    const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
+    fused_tile<R>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
+                  reinterpret_cast<const FOp*>(prog + it.op_off), reinterpret_cast<const double*>(prog + it.coef_off),
+                  blockIdx.x - it.block0, reinterpret_cast<cplx<R>*>(smem));
+}
+
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
@@ -215,6 +239,20 @@ Layout layout_of(const Program& prog) {
 }  // namespace
 
 size_t program_bytes(const Program& prog) { return layout_of(prog).total; }
+
+int launch_batch(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const BItem* items, const uint32_t* map,
+                 uint32_t t, uint32_t blocks) {
+    const size_t amp = prec == QVNT_F64 ? 16 : 8;
+    const dim3 grid(blocks), block(1u << (t - kRegBits));
+    const size_t lds = ((size_t)1 << t) * amp;
+    const unsigned char* p = static_cast<const unsigned char*>(prog);
+    if (prec == QVNT_F64)
+        hipLaunchKernelGGL(k_batch<double>, grid, block, lds, stream, static_cast<c64*>(arena), p, items, map);
+    else
+        hipLaunchKernelGGL(k_batch<float>, grid, block, lds, stream, static_cast<c32*>(arena), p, items, map);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
 
 void pack_program(const Program& prog, unsigned char* host) {
     const Layout L = layout_of(prog);

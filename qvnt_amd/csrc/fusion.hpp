@@ -68,6 +68,18 @@ struct FPass {
     uint64_t hoff[kRegs];     // global offset of local bits t-4..t-1 = j
 };
 
+// One register's pass in a batched launch: byte offsets into the program
+// buffer and the register's amplitude offset in the batch arena.
+struct BItem {              // 32 bytes
+    uint64_t psi_off;       // amplitudes
+    uint32_t pass_off;      // bytes: FPass
+    uint32_t phase_off;     // bytes: the register program's FPhase array
+    uint32_t op_off;        // bytes: its FOp array
+    uint32_t coef_off;      // bytes: its coefficients
+    uint32_t block0;        // first block of the item in its launch
+    uint32_t pad;
+};
+
 // LDS swizzle: bits 4..7 of the local index are XORed into bits 0..3, so
 // 16 lanes that differ in bits 4..7 hit 16 different 16-byte bank groups.
 inline uint32_t lds_swizzle(uint32_t e) { return e ^ ((e >> 4) & 15u); }

@@ -85,5 +85,9 @@ namespace fusion { struct Program; }
 size_t program_bytes(const fusion::Program& prog);
 void pack_program(const fusion::Program& prog, unsigned char* host);
 int launch_program(qvnt_reg* r, const fusion::Program& prog, const void* dev);
+namespace fusion { struct BItem; }
+// one batched launch: `blocks` tiles of 2^t amplitudes (block -> item by map)
+int launch_batch(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const fusion::BItem* items,
+                 const uint32_t* map, uint32_t t, uint32_t blocks);
 
 }  // namespace qvnt

@@ -1,0 +1,136 @@
+"""The batched-register form: every register's result equals the
+single-register fused form bit for bit (same tile cap), the plain form and
+the oracle to 1e-12 (FP64), and stays within the FP32 bound in FP32."""
+import numpy as np
+import pytest
+
+import qvnt_amd
+from qvnt_amd import QReg, circuits, op
+from qvnt_amd.batch import TUNE_CAPS, Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def workload(count, seed, n_lo=6, n_hi=14, g_lo=20, g_hi=200):
+    regs = circuits.batched_registers(count, seed=seed, n_lo=n_lo, n_hi=n_hi, g_lo=g_lo, g_hi=g_hi)
+    states = np.random.default_rng(seed).integers(0, 1 << 62, count, dtype=np.uint64)
+    states = [int(s) & ((1 << n) - 1) for s, (n, _) in zip(states, regs)]
+    return regs, states, [circuits.build(specs, op) for _, specs in regs]
+
+
+def infidelity(a, b):
+    return 1.0 - abs(np.vdot(a.astype(np.complex128), b.astype(np.complex128))) ** 2
+
+
+@pytest.mark.parametrize("cap", [6, 9, 12])
+def test_batch_equals_single_fused(cap):
+    regs, states, ops = workload(40, 100 + cap)
+    with Batch([n for n, _ in regs], states) as b:
+        info = b.compile(ops, max_qubits=cap)
+        b.run()
+        assert info["single_ops"] == sum(len(o) for o in ops)
+        for k, (n, _) in enumerate(regs):
+            with QReg.with_state(n, states[k]) as r:
+                r.apply_fused(ops[k], max_qubits=cap)
+                want = r.get_state()
+            assert np.array_equal(b.get_state(k), want[: 1 << n]), "register %d (%d qubits)" % (k, n)
+
+
+def test_batch_matches_plain_and_oracle(orc):
+    regs, states, ops = workload(24, 7, n_lo=6, n_hi=12)
+    with Batch([n for n, _ in regs], states) as b:
+        b.compile(ops, max_qubits=10)
+        b.run()
+        for k, (n, specs) in enumerate(regs):
+            with QReg.with_state(n, states[k]) as r:
+                r.apply(ops[k])
+                plain = r.get_state()[: 1 << n]
+            got = b.get_state(k)
+            assert np.max(np.abs(got - plain)) <= 1e-12
+            if k < 8:
+                ref = orc.Reg(n, states[k])
+                ref.apply(circuits.build(specs, orc.op))
+                assert np.max(np.abs(got - ref.state()[: 1 << n])) <= 1e-12
+
+
+def test_autotune_and_reset():
+    regs, states, ops = workload(48, 3, n_lo=8, n_hi=14)
+    with Batch([n for n, _ in regs], states) as b:
+        info = b.compile(ops)                 # autotuner: runs the batch, then resets it
+        for n, cap in info["tile_cap"].items():
+            assert cap in TUNE_CAPS
+            assert info["tune_ms"][n], "qubit count %d was not timed" % n
+        b.run()
+        got = [b.get_state(k) for k in range(len(regs))]
+    for k, (n, _) in enumerate(regs):
+        with QReg.with_state(n, states[k]) as r:
+            r.apply(ops[k])
+            assert np.max(np.abs(got[k] - r.get_state()[: 1 << n])) <= 1e-12
+
+
+def test_graph_replays_match_eager():
+    regs, states, ops = workload(32, 5, n_lo=6, n_hi=13)
+    with Batch([n for n, _ in regs], states) as b:
+        b.compile(ops, max_qubits=9)
+        b.run(graph=False)
+        eager = [b.get_state(k) for k in range(len(regs))]
+        for rep in range(3):
+            b.reset()
+            b.run(graph=True)
+            b.sync()
+            for k in range(len(regs)):
+                assert np.array_equal(b.get_state(k), eager[k]), "replay %d, register %d" % (rep, k)
+            # eager work between replays
+            b.reset()
+            b.run(graph=False)
+            assert np.array_equal(b.get_state(0), eager[0])
+        # two runs in a row apply the circuits twice
+        b.reset()
+        b.run(graph=True)
+        b.run(graph=True)
+        twice = b.get_state(1)
+    n = regs[1][0]
+    with QReg.with_state(n, states[1]) as r:
+        r.apply_fused(ops[1], max_qubits=9)
+        r.apply_fused(ops[1], max_qubits=9)
+        assert np.array_equal(twice, r.get_state()[: 1 << n])
+
+
+def test_fp32_batch_error_bound():
+    regs, states, ops = workload(24, 9, n_lo=8, n_hi=14)
+    with Batch([n for n, _ in regs], states, prec="f32") as b:
+        b.compile(ops, max_qubits=12)
+        b.run()
+        for k, (n, _) in enumerate(regs):
+            with QReg.with_state(n, states[k]) as r:
+                r.apply(ops[k])
+                ref = r.get_state()[: 1 << n]
+            got = b.get_state(k)
+            assert got.dtype == np.complex64
+            assert np.max(np.abs(got - ref)) <= 1e-5
+            assert infidelity(got, ref) <= 1e-5
+
+
+def test_batch_sampling_matches_register():
+    regs, states, ops = workload(6, 11, n_lo=10, n_hi=12)
+    draws = np.array([qvnt_amd.uniform(11, 0, j) for j in range(512)])
+    with Batch([n for n, _ in regs], states) as b:
+        b.compile(ops, max_qubits=8)
+        b.run()
+        for k, (n, _) in enumerate(regs):
+            psi = b.get_state(k)
+            outs, total = b.sample_mask(k, 0xF0, draws)
+            with QReg(n) as r:
+                r.set_state(psi)
+                assert np.array_equal(outs, r.sample_mask(0xF0, draws))
+                assert total == r.get_absolute()
+
+
+def test_batch_rejects_bad_input():
+    with pytest.raises(qvnt_amd.QvntError):
+        Batch([5, 8])
+    with Batch([6, 7]) as b:
+        with pytest.raises(qvnt_amd.QvntError):
+            b.compile([op.x(1), op.x(1 << 7)])     # qubit 7 beyond a 7-qubit register
+        with pytest.raises(TypeError):
+            b.compile([op.x(1)])
