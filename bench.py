@@ -59,12 +59,14 @@ def cpu_baseline(n, budget_s):
                       "out-of-place gather + per-apply copy), %.1f s" % (done, n, elapsed)}
 
 
-def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value):
+def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value, dense=0):
     """The same layers through qvnt_reg_apply_fused, one call per layer,
-    then the whole timed circuit as one call."""
+    then the whole timed circuit as one call (dense: the dense-phase
+    threshold, 0 for none)."""
     out = {}
     for label, groups in (("per_layer", [[o] for o in ops[warmup:]]), ("whole_circuit", [ops[warmup:]])):
         reg = QReg(n)
+        reg.set_dense(dense)
         for o in ops[:warmup]:
             reg.apply_fused(o)
         merged = []
@@ -318,11 +320,11 @@ def batch(args):
         log("plain: %.2f s -> %.0f SingleOps/s" % (wall, single / wall))
     with Batch([n for n, _ in regs], prec=prec) as b:
         t0 = time.perf_counter()
-        info = b.compile(ops, max_qubits=args.tile_cap)
+        info = b.compile(ops, max_qubits=args.tile_cap, dense_min=args.dense)
         comp = time.perf_counter() - t0
-        log("compile%s: %.2f s, %d rounds, %d launches, %d passes; caps %s" % (
+        log("compile%s: %.2f s, %d rounds, %d launches, %d passes, %d of %d phases dense; caps %s; dense %s" % (
             " + autotune" if not args.tile_cap else "", comp, info["rounds"], info["launches"], info["passes"],
-            info["tile_cap"]))
+            info["dense_phases"], info["phases"], info["tile_cap"], info["dense_min"]))
         b.run(graph=True)
         b.sync()
         errs = []
@@ -341,7 +343,9 @@ def batch(args):
     out["batched"] = {"value": round(value, 1), "ms_per_run_graph": round(ms_graph, 3),
                       "ms_per_run_eager": round(ms_eager, 3), "compile_s": round(comp, 2),
                       "rounds": info["rounds"], "launches": info["launches"], "passes": info["passes"],
-                      "tiles": info["tiles"], "tile_cap": info["tile_cap"], "tune_ms": info["tune_ms"],
+                      "tiles": info["tiles"], "phases": info["phases"], "dense_phases": info["dense_phases"],
+                      "tile_cap": info["tile_cap"], "dense_min": info["dense_min"], "tune_ms": info["tune_ms"],
+                      "dense_ms": info["dense_ms"],
                       "max_abs_err_vs_plain": max(errs) if errs else None, "checked_registers": len(errs)}
     line = {"metric": METRIC, "value": round(value, 1), "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True, "scaling": "weak",
@@ -379,6 +383,9 @@ def main():
     ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
+    ap.add_argument("--dense", type=int, default=0,
+                    help="fused forms: fold phases of >= this many ops into dense 16x16 unitaries (0: none; "
+                         "batch: used with --tile-cap, else autotuned)")
     ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
@@ -414,7 +421,7 @@ def main():
     timed_gates = sum(len(o) for o in ops[args.warmup:])
 
     if args.no_plain:
-        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, None)
+        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, None, args.dense)
         print(json.dumps({"fused": fused}), flush=True)
         return
 
@@ -456,7 +463,7 @@ def main():
 
     fused = None
     if not args.no_fused:
-        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, value)
+        fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, value, args.dense)
 
     cpu = None
     if not args.no_cpu:

@@ -283,6 +283,11 @@ typedef struct qvnt_kstats {
     double   bytes;    /* summed algorithmic bytes 2*s*2^(n-popcount(ctrl)) */
 } qvnt_kstats;
 int qvnt_reg_set_timing(qvnt_reg* reg, int enable);
+/* Fused form (qvnt_reg_apply_fused): a phase of at least min_ops primitives
+ * on four register qubits only, in tiles of >= 2^10 amplitudes, is folded
+ * into one dense 16x16 unitary applied on the matrix cores (0: never, the
+ * default).  No counterpart in the reference. */
+int qvnt_reg_set_dense(qvnt_reg* reg, uint32_t min_ops);
 /* Synchronises, folds pending events into the totals, copies
  * QVNT_KCLASS_COUNT entries into out, and (if reset) clears them. */
 int qvnt_reg_kernel_stats(qvnt_reg* reg, qvnt_kstats* out, int reset);
@@ -301,9 +306,15 @@ typedef struct {
     uint64_t tiles;           /* tiles (workgroups) per run */
     uint64_t single_ops;      /* reference SingleOps of all circuits */
     double bytes;             /* algorithmic HBM bytes per run: 2 * amp * 2^n per pass */
+    uint64_t phases;          /* phases per run (all passes) */
+    uint64_t dense_phases;    /* of which dense 16x16 phases on the matrix cores */
     uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
+    uint32_t dense_min[32];   /* dense-phase threshold used for each qubit count (0: none) */
     double tune_ms[32][5];    /* autotuner: ms per run of the registers of each qubit count
-                                 with caps 8, 9, 10, 11, 12 (0: not timed / same as a smaller cap) */
+                                 with caps 8, 9, 10, 11, 12, no dense phases (0: not timed or
+                                 the same as a smaller cap) */
+    double dense_ms[32][4];   /* autotuner: the same at the chosen cap with dense thresholds
+                                 2, 3, 4, 6 (0: not timed) */
 } qvnt_batch_info;
 /* Registers k = 0..count-1 of q_nums[k] qubits, each set to basis state
  * states[k] (null: all 0). */
@@ -312,10 +323,13 @@ uint32_t qvnt_batch_count(const qvnt_batch* b);
 /* Every register back to its initial basis state (stream-ordered). */
 int qvnt_batch_reset(qvnt_batch* b);
 /* Compiles ops[k] (queue order, as qvnt_reg_apply) for register k.
- * max_qubits > 0 caps every tile at that many qubits (6..12); 0 runs the
- * autotuner, which times every cap on the registers of each size and
- * leaves the registers reset to their initial states. */
-int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, qvnt_batch_info* info);
+ * max_qubits > 0 caps every tile at that many qubits (6..12) and folds
+ * phases of >= dense_min ops into dense 16x16 unitaries (0: none; see
+ * qvnt_reg_set_dense).  max_qubits = 0 runs the autotuner instead: for the
+ * registers of each size it times every tile cap, then dense thresholds at
+ * the best cap, and it leaves the registers reset to their initial states. */
+int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, uint32_t dense_min,
+                       qvnt_batch_info* info);
 /* Queues one application of every compiled circuit (graph != 0: as one
  * hipGraph, captured at the first such call after a compile). */
 int qvnt_batch_run(qvnt_batch* b, int graph);

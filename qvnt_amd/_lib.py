@@ -97,6 +97,7 @@ def _declare(L):
         "qvnt_reg_set_state": ([vp, vp, sz], i),
         "qvnt_reg_sync": ([vp], i),
         "qvnt_reg_set_timing": ([vp, i], i),
+        "qvnt_reg_set_dense": ([vp, u32], i),
         "qvnt_reg_kernel_stats": ([vp, ctypes.POINTER(KStats), i], i),
     }
     for name in ["x", "y", "z", "s", "t", "swap", "sqrt_swap", "i_swap", "sqrt_i_swap", "qft", "qft_swapped"]:

@@ -12,10 +12,13 @@ from .ops import MultiOp
 class BatchInfo(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_uint32), ("launches", ctypes.c_uint32), ("passes", ctypes.c_uint64),
                 ("tiles", ctypes.c_uint64), ("single_ops", ctypes.c_uint64), ("bytes", ctypes.c_double),
-                ("tile_cap", ctypes.c_uint32 * 32), ("tune_ms", (ctypes.c_double * 5) * 32)]
+                ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64),
+                ("tile_cap", ctypes.c_uint32 * 32), ("dense_min", ctypes.c_uint32 * 32),
+                ("tune_ms", (ctypes.c_double * 5) * 32), ("dense_ms", (ctypes.c_double * 4) * 32)]
 
 
 TUNE_CAPS = (8, 9, 10, 11, 12)
+TUNE_DENSE = (2, 3, 4, 6)
 
 
 def _declare():
@@ -25,7 +28,7 @@ def _declare():
         "qvnt_batch_new": ([u32, P(u32), P(u64), u32, P(vp)], i),
         "qvnt_batch_count": ([vp], u32),
         "qvnt_batch_reset": ([vp], i),
-        "qvnt_batch_compile": ([vp, P(vp), u32, P(BatchInfo)], i),
+        "qvnt_batch_compile": ([vp, P(vp), u32, u32, P(BatchInfo)], i),
         "qvnt_batch_run": ([vp, i], i),
         "qvnt_batch_time": ([vp, u32, i, P(d)], i),
         "qvnt_batch_sync": ([vp], i),
@@ -59,22 +62,30 @@ class Batch:
     def __len__(self):
         return len(self.q_nums)
 
-    def compile(self, ops, max_qubits=0):
-        """Compiles ops[k] for register k; max_qubits = 0 runs the autotuner
-        (the registers are reset afterwards).  Returns the schedule summary."""
+    def compile(self, ops, max_qubits=0, dense_min=0):
+        """Compiles ops[k] for register k with tiles of at most max_qubits
+        qubits and dense phases of >= dense_min ops (0: none); max_qubits = 0
+        runs the autotuner instead (the registers are reset afterwards).
+        Returns the schedule summary."""
         if len(ops) != len(self.q_nums) or not all(isinstance(o, MultiOp) for o in ops):
             raise TypeError("compile expects one MultiOp per register")
         arr = (ctypes.c_void_p * len(ops))(*[o.handle for o in ops])
         info = BatchInfo()
-        check(lib.qvnt_batch_compile(self._h, arr, max_qubits, ctypes.byref(info)))
-        tune = {}
-        for n in sorted(set(self.q_nums)):
+        check(lib.qvnt_batch_compile(self._h, arr, max_qubits, dense_min, ctypes.byref(info)))
+        sizes = sorted(set(self.q_nums))
+        tune, dtune = {}, {}
+        for n in sizes:
             row = {c: info.tune_ms[n][i] for i, c in enumerate(TUNE_CAPS) if info.tune_ms[n][i] > 0}
             if row:
                 tune[n] = row
+            row = {d: info.dense_ms[n][i] for i, d in enumerate(TUNE_DENSE) if info.dense_ms[n][i] > 0}
+            if row:
+                dtune[n] = row
         return {"rounds": info.rounds, "launches": info.launches, "passes": info.passes, "tiles": info.tiles,
-                "single_ops": info.single_ops, "bytes": info.bytes,
-                "tile_cap": {n: info.tile_cap[n] for n in sorted(set(self.q_nums))}, "tune_ms": tune}
+                "single_ops": info.single_ops, "bytes": info.bytes, "phases": info.phases,
+                "dense_phases": info.dense_phases,
+                "tile_cap": {n: info.tile_cap[n] for n in sizes}, "dense_min": {n: info.dense_min[n] for n in sizes},
+                "tune_ms": tune, "dense_ms": dtune}
 
     def reset(self):
         check(lib.qvnt_batch_reset(self._h))

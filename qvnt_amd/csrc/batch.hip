@@ -34,6 +34,7 @@ namespace {
 constexpr unsigned kMinQubits = qvnt::fusion::kMinFused;
 constexpr unsigned kMaxQubits = 30;
 constexpr unsigned kTuneCaps[] = {8, 9, 10, 11, 12};
+constexpr unsigned kTuneDense[] = {2, 3, 4, 6};
 
 size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -95,7 +96,7 @@ struct Launch {
 struct Schedule {
     std::vector<Launch> launches;
     uint32_t rounds = 0;
-    uint64_t passes = 0, tiles = 0;
+    uint64_t passes = 0, tiles = 0, phases = 0, dense_phases = 0;
     double bytes = 0.0;    // algorithmic HBM bytes of one run: 2 * amp * 2^n per pass
     void* dprog = nullptr;
     void* dlaunch = nullptr;
@@ -154,10 +155,12 @@ int reset(qvnt_batch* b) {
     return QVNT_OK;
 }
 
-// Compiles the circuits of registers `sel` with tile cap caps[q] (parallel
-// over host threads).  All gates were validated by the caller.
+// Compiles the circuits of registers `sel` with tile cap caps[q] and dense
+// threshold dense[q] (parallel over host threads).  All gates were
+// validated by the caller.
 void compile_plans(const std::vector<std::vector<qvnt_gate>>& gates, const std::vector<uint32_t>& q,
-                   const std::vector<uint32_t>& sel, const uint32_t* caps, std::vector<RegPlan>& plans, int* bad) {
+                   const std::vector<uint32_t>& sel, const uint32_t* caps, const uint32_t* dense,
+                   std::vector<RegPlan>& plans, int* bad) {
     std::atomic<size_t> next{0};
     std::atomic<int> err{-1};
     auto work = [&] {
@@ -168,7 +171,7 @@ void compile_plans(const std::vector<std::vector<qvnt_gate>>& gates, const std::
             const unsigned ns = q[k];
             const unsigned tq = std::min<unsigned>(caps[ns], ns);
             std::vector<Step> steps;
-            qvnt::fusion::plan(gates[k].data(), gates[k].size(), ns, tq, steps);
+            qvnt::fusion::plan(gates[k].data(), gates[k].size(), ns, tq, steps, dense[ns]);
             RegPlan& rp = plans[k];
             rp.prog = Program();
             // an all-fusable queue compiles to one program; a gate the fused
@@ -192,7 +195,7 @@ void compile_plans(const std::vector<std::vector<qvnt_gate>>& gates, const std::
 int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::vector<uint32_t>& sel, Schedule& S) {
     S.release();
     S.rounds = 0;
-    S.passes = S.tiles = 0;
+    S.passes = S.tiles = S.phases = S.dense_phases = 0;
     S.bytes = 0.0;
     size_t total = 0;
     std::vector<size_t> base(b->count, 0);
@@ -238,6 +241,8 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
                 items.push_back(it);
                 S.passes += 1;
                 S.tiles += tiles;
+                S.phases += fp.phase_end - fp.phase_begin;
+                for (uint32_t ph = fp.phase_begin; ph < fp.phase_end; ++ph) S.dense_phases += plans[k].prog.phases[ph].dense;
                 S.bytes += 2.0 * (double)b->amp() * (double)(1ull << b->q[k]);
             }
             L.blocks = (uint32_t)map.size();
@@ -364,7 +369,8 @@ int qvnt_batch_reset(qvnt_batch* b) {
 
 uint32_t qvnt_batch_count(const qvnt_batch* b) { return b ? b->count : 0; }
 
-int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, qvnt_batch_info* info) {
+int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, uint32_t dense_min,
+                       qvnt_batch_info* info) {
     if (int rc = check_batch(b)) return rc;
     if (!ops) return fail(QVNT_E_ARG, "null operator array");
     std::vector<std::vector<qvnt_gate>> gates(b->count);
@@ -385,43 +391,81 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
     std::vector<RegPlan> plans(b->count);
     std::vector<uint32_t> all(b->count);
     for (uint32_t k = 0; k < b->count; ++k) all[k] = k;
-    uint32_t caps[kMaxQubits + 1];
-    double tune_ms[kMaxQubits + 1][sizeof(kTuneCaps) / sizeof(kTuneCaps[0])] = {};
+    constexpr size_t kCaps = sizeof(kTuneCaps) / sizeof(kTuneCaps[0]);
+    constexpr size_t kDense = sizeof(kTuneDense) / sizeof(kTuneDense[0]);
+    uint32_t caps[kMaxQubits + 1], dense[kMaxQubits + 1];
+    double tune_ms[kMaxQubits + 1][kCaps] = {}, dense_ms[kMaxQubits + 1][kDense] = {};
     int bad = -1;
+    auto no_batched_form = [&] {
+        return fail(QVNT_E_ARG, "register " + std::to_string(bad) + " holds a gate with no batched form");
+    };
     if (max_qubits) {
         const unsigned c = std::max(kMinQubits, std::min<unsigned>(max_qubits, qvnt::fusion::kMaxTile));
-        for (unsigned n = 0; n <= kMaxQubits; ++n) caps[n] = c;
+        for (unsigned n = 0; n <= kMaxQubits; ++n) {
+            caps[n] = c;
+            dense[n] = dense_min;
+        }
     } else {
-        // autotune: time every candidate cap on the registers of each size
+        // autotune, per register size: every tile cap without dense phases,
+        // then dense thresholds at the best cap; each candidate is timed on
+        // the batch's own registers of that size
         std::map<uint32_t, std::vector<uint32_t>> by_q;
         for (uint32_t k = 0; k < b->count; ++k) by_q[b->q[k]].push_back(k);
-        for (unsigned n = 0; n <= kMaxQubits; ++n) caps[n] = qvnt::fusion::kMaxTile;
         std::vector<double> best(kMaxQubits + 1, 1e300);
-        for (size_t ci = 0; ci < sizeof(kTuneCaps) / sizeof(kTuneCaps[0]); ++ci) {
-            uint32_t trial[kMaxQubits + 1];
-            for (unsigned n = 0; n <= kMaxQubits; ++n) trial[n] = kTuneCaps[ci];
-            compile_plans(gates, b->q, all, trial, plans, &bad);
-            if (bad >= 0) return fail(QVNT_E_ARG, "register " + std::to_string(bad) + " holds a gate with no batched form");
+        for (unsigned n = 0; n <= kMaxQubits; ++n) {
+            caps[n] = qvnt::fusion::kMaxTile;
+            dense[n] = 0;
+        }
+        auto trial_run = [&](const uint32_t* tc, const uint32_t* td, auto&& want, auto&& record) -> int {
+            compile_plans(gates, b->q, all, tc, td, plans, &bad);
+            if (bad >= 0) return no_batched_form();
             for (auto& kv : by_q) {
-                const unsigned n = kv.first;
-                if (kTuneCaps[ci] > n && ci > 0 && kTuneCaps[ci - 1] >= n) continue;  // same effective cap as before
+                if (!want(kv.first)) continue;
                 Schedule S;
                 if (int rc = build_schedule(b, plans, kv.second, S)) return rc;
                 double ms = 0.0;
                 const int rc = time_schedule(b, S, 2, &ms);
                 S.release();
                 if (rc) return rc;
-                tune_ms[n][ci] = ms;
-                if (ms < best[n]) {
-                    best[n] = ms;
-                    caps[n] = kTuneCaps[ci];
-                }
+                record(kv.first, ms);
             }
+            return QVNT_OK;
+        };
+        uint32_t tc[kMaxQubits + 1], td[kMaxQubits + 1];
+        for (size_t ci = 0; ci < kCaps; ++ci) {
+            for (unsigned n = 0; n <= kMaxQubits; ++n) {
+                tc[n] = kTuneCaps[ci];
+                td[n] = 0;
+            }
+            const int rc = trial_run(
+                tc, td,
+                [&](unsigned n) { return !(kTuneCaps[ci] > n && ci > 0 && kTuneCaps[ci - 1] >= n); },  // same effective cap
+                [&](unsigned n, double ms) {
+                    tune_ms[n][ci] = ms;
+                    if (ms < best[n]) {
+                        best[n] = ms;
+                        caps[n] = kTuneCaps[ci];
+                    }
+                });
+            if (rc) return rc;
+        }
+        for (size_t di = 0; di < kDense; ++di) {
+            for (unsigned n = 0; n <= kMaxQubits; ++n) td[n] = kTuneDense[di];
+            const int rc = trial_run(
+                caps, td, [&](unsigned n) { return std::min(caps[n], n) >= (unsigned)qvnt::fusion::kMinDenseTile; },
+                [&](unsigned n, double ms) {
+                    dense_ms[n][di] = ms;
+                    if (ms < best[n]) {
+                        best[n] = ms;
+                        dense[n] = kTuneDense[di];
+                    }
+                });
+            if (rc) return rc;
         }
         if (int rc = reset(b)) return rc;
     }
-    compile_plans(gates, b->q, all, caps, plans, &bad);
-    if (bad >= 0) return fail(QVNT_E_ARG, "register " + std::to_string(bad) + " holds a gate with no batched form");
+    compile_plans(gates, b->q, all, caps, dense, plans, &bad);
+    if (bad >= 0) return no_batched_form();
     if (int rc = build_schedule(b, plans, all, b->sched)) return rc;
     std::memcpy(b->caps, caps, sizeof(caps));
     b->single_ops = single;
@@ -433,9 +477,13 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         info->tiles = b->sched.tiles;
         info->bytes = b->sched.bytes;
         info->single_ops = single;
+        info->phases = b->sched.phases;
+        info->dense_phases = b->sched.dense_phases;
         for (unsigned n = 0; n <= kMaxQubits && n < 32; ++n) {
             info->tile_cap[n] = caps[n];
-            for (size_t ci = 0; ci < 5; ++ci) info->tune_ms[n][ci] = tune_ms[n][ci];
+            info->dense_min[n] = dense[n];
+            for (size_t ci = 0; ci < kCaps; ++ci) info->tune_ms[n][ci] = tune_ms[n][ci];
+            for (size_t di = 0; di < kDense; ++di) info->dense_ms[n][di] = dense_ms[n][di];
         }
     }
     return QVNT_OK;

@@ -137,6 +137,80 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
     }
 }
 
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step: acc(16x16) += A(16x4) * B(4x16).
+// Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].  Element i of
+// its accumulator is acc[l / 16 + 4 i][l % 16] for f64 and
+// acc[4 (l / 16) + i][l % 16] for f32 (measured on gfx950).
+__device__ __forceinline__ mf64x4 mfma(double a, double b, mf64x4 acc) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ mf32x4 mfma(float a, float b, mf32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+template <class R> struct AccOf;
+template <> struct AccOf<double> { typedef mf64x4 type; static constexpr bool hi_low_bits = true; };
+template <> struct AccOf<float> { typedef mf32x4 type; static constexpr bool hi_low_bits = false; };
+
+// swizzled LDS offset of register-index bits (j & 3) placed at register
+// bits (b0, b0 + 1) of the phase: the swizzle is XOR-linear, so offsets of
+// disjoint bit groups combine by XOR
+__device__ __forceinline__ uint32_t reg_off2(const FPhase& ph, int b0, uint32_t j) {
+    return dswz(((j & 1u) << ph.rpos[b0]) | (((j >> 1) & 1u) << ph.rpos[b0 + 1]));
+}
+
+// A dense phase: the tile (in LDS) is a 16 x 2^(t-4) complex matrix X
+// (row = register index of the phase, column = thread base), and the phase
+// computes X := U X as a real 32x32 GEMM on the matrix cores:
+//   Yre = Ure Xre - Uim Xim,   Yim = Ure Xim + Uim Xre.
+// Each 64-lane wave owns the 64 columns of its own threads (as in the
+// register phases), in four 16-column tiles; all reads of a tile's columns
+// precede its writes, so the update is in place.
+template <class R>
+__device__ __forceinline__ void dense_phase(cplx<R>* lds, const FPhase& ph, const double* __restrict__ u, uint32_t tid) {
+    typedef typename AccOf<R>::type Acc;
+    const uint32_t lane = tid & 63u, hi = lane >> 4, wave0 = tid & ~63u;
+    // A operands: U[m][4s + hi] with m = lane % 16
+    R ure[4], uim[4], nuim[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int e = 16 * (int)(lane & 15u) + 4 * s + (int)hi;
+        ure[s] = (R)u[2 * e];
+        uim[s] = (R)u[2 * e + 1];
+        nuim[s] = -uim[s];
+    }
+    const uint32_t in_hi = reg_off2(ph, 0, hi);   // B rows 4s + hi: register bits 0,1 = hi
+    // accumulator rows: hi + 4i (f64) or 4 hi + i (f32)
+    constexpr bool lowhi = AccOf<R>::hi_low_bits;
+    const uint32_t out_hi = reg_off2(ph, lowhi ? 0 : 2, hi);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        // this lane's column and its LDS base (zeros at the register bits)
+        uint32_t tb = wave0 + 16u * c + (lane & 15u);
+#pragma unroll
+        for (int k = 0; k < kRegBits; ++k) {
+            const uint32_t lo = tb & ((1u << ph.rpos[k]) - 1u);
+            tb = ((tb ^ lo) << 1) | lo;
+        }
+        const uint32_t stb = dswz(tb);
+        cplx<R> x[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) x[s] = lds[stb ^ in_hi ^ reg_off2(ph, 2, (uint32_t)s)];
+        Acc yre = {0, 0, 0, 0}, yim = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            yre = mfma(ure[s], x[s].re, yre);
+            yim = mfma(ure[s], x[s].im, yim);
+            yre = mfma(nuim[s], x[s].im, yre);
+            yim = mfma(uim[s], x[s].re, yim);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lds[stb ^ out_hi ^ reg_off2(ph, lowhi ? 2 : 0, (uint32_t)i)] = cplx<R>{yre[i], yim[i]};
+    }
+}
+
 // One tile of one pass: read 2^t amplitudes, run the pass's phases, write
 // them back.  `tile_index` enumerates the tiles (the bits outside the
 // tile's qubit set); the pass and program live in scalar-loadable memory.
@@ -166,6 +240,22 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
     const uint32_t p0 = pass.phase_begin, p1 = pass.phase_end;
     for (uint32_t p = p0; p < p1; ++p) {
         const FPhase& ph = phases[p];
+        if (ph.dense) {
+            // works on the LDS tile: bring the HBM registers in first, and
+            // take the result back out for the final store
+            if (p == p0) {
+#pragma unroll
+                for (int j = 0; j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
+                __syncthreads();
+            }
+            dense_phase<R>(lds, ph, coef + ph.ucoef, tid);
+            __syncthreads();
+            if (p + 1 == p1) {
+#pragma unroll
+                for (int j = 0; j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
+            }
+            continue;
+        }
         // local base of this thread in the phase: zeros at the register bits
         uint32_t tb = tid;
 #pragma unroll
@@ -319,7 +409,7 @@ int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, 
     if (tq < (unsigned)kMinFused) tq = kMinFused;
     if (tq > r->ns) tq = r->ns;
     std::vector<Step> steps;
-    plan(g, n, r->ns, tq, steps);
+    plan(g, n, r->ns, tq, steps, r->dense_min);
     uint32_t count = 0;
     for (const Step& st : steps) {
         if (st.plain) {

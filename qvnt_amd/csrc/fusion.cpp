@@ -275,7 +275,53 @@ FPhase make_phase(uint32_t Rl) {
     return ph;
 }
 
-void compile_pass(Program& prog, const PassPlan& pp) {
+// The 16x16 action on the register index of the ops [op_begin, op_end) of
+// a phase whose ops touch register bits only, in the kernel's semantics
+// (fused.hip: apply_m2 / apply_m4 / apply_d with table variant 0).
+void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
+    for (int r = 0; r < kRegs; ++r)
+        for (int c = 0; c < kRegs; ++c) U[r * kRegs + c] = r == c ? 1.0 : 0.0;
+    for (uint32_t o = op_begin; o < op_end; ++o) {
+        const FOp& op = prog.ops[o];
+        auto cf = [&](int k) { return cd(prog.coef[op.coef + 2 * k], prog.coef[op.coef + 2 * k + 1]); };
+        cd G[kRegs * kRegs];
+        for (int r = 0; r < kRegs; ++r)
+            for (int c = 0; c < kRegs; ++c) G[r * kRegs + c] = r == c ? 1.0 : 0.0;
+        if (op.type == OP_D) {
+            for (int i = 0; i < kRegs; ++i) G[i * kRegs + i] = cf(i);
+        } else if (op.type == OP_M2) {
+            const int J = (int)op.sel;
+            for (int i = 0; i < kRegs; ++i) {
+                if (i & (1 << J)) continue;
+                if (op.pc != 0xFFFFu && !((op.pc >> i) & 1u)) continue;
+                const int i1 = i | (1 << J);
+                G[i * kRegs + i] = cf(0);
+                G[i * kRegs + i1] = cf(1);
+                G[i1 * kRegs + i] = cf(2);
+                G[i1 * kRegs + i1] = cf(3);
+            }
+        } else {
+            const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;
+            for (int i = 0; i < kRegs; ++i) {
+                if (i & ((1 << j0) | (1 << j1))) continue;
+                if (!((op.pc >> i) & 1u)) continue;
+                const int idx[4] = {i, i | (1 << j0), i | (1 << j1), i | (1 << j0) | (1 << j1)};
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) G[idx[r] * kRegs + idx[c]] = cf(4 * r + c);
+            }
+        }
+        cd T[kRegs * kRegs];
+        for (int r = 0; r < kRegs; ++r)
+            for (int c = 0; c < kRegs; ++c) {
+                cd acc = 0.0;
+                for (int k = 0; k < kRegs; ++k) acc += G[r * kRegs + k] * U[k * kRegs + c];
+                T[r * kRegs + c] = acc;
+            }
+        for (int k = 0; k < kRegs * kRegs; ++k) U[k] = T[k];
+    }
+}
+
+void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     FPass fp{};
     fp.t = popc(pp.T);
     fp.nthreads = 1u << (fp.t - kRegBits);
@@ -321,6 +367,8 @@ void compile_pass(Program& prog, const PassPlan& pp) {
             return -1;
         };
         ph.op_begin = (uint32_t)prog.ops.size();
+        const size_t coef_begin = prog.coef.size();
+        bool foldable = dense_min > 0 && fp.t >= (uint32_t)kMinDenseTile && take.size() >= dense_min;
         for (const Prim& p : take) {
             FOp op{};
             op.type = p.type;
@@ -376,14 +424,25 @@ void compile_pass(Program& prog, const PassPlan& pp) {
                 op.sel = (uint32_t)(4 * j0 + j1);
                 op.coef = add_coef(prog, u, 16);
             }
+            if (op.lctrl || op.gctrl || op.la || op.lb || op.ga || op.gb) foldable = false;
             prog.ops.push_back(op);
         }
         ph.op_end = (uint32_t)prog.ops.size();
+        if (foldable) {
+            cd U[kRegs * kRegs];
+            dense_of(prog, ph.op_begin, ph.op_end, U);
+            prog.ops.resize(ph.op_begin);
+            prog.coef.resize(coef_begin);
+            ph.op_end = ph.op_begin;
+            ph.dense = 1;
+            ph.ucoef = add_coef(prog, U, kRegs * kRegs);
+        }
         prog.phases.push_back(ph);
         rest.swap(leave);
     }
     fp.phase_end = (uint32_t)prog.phases.size();
     auto is_top = [&](const FPhase& ph) {
+        if (ph.dense) return false;  // dense phases work from the LDS tile
         for (int k = 0; k < kRegBits; ++k)
             if (ph.rpos[k] != fp.t - kRegBits + k) return false;
         return true;
@@ -395,7 +454,7 @@ void compile_pass(Program& prog, const PassPlan& pp) {
 
 }  // namespace
 
-void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps) {
+void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min) {
     const uint64_t lmask = (1ull << ns) - 1u;
     size_t i = 0;
     while (i < n) {
@@ -408,7 +467,7 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
         }
         if (!prims.empty()) {
             Step st;
-            for (const PassPlan& pp : schedule_passes(merge_single(prims), ns, tq)) compile_pass(st.prog, pp);
+            for (const PassPlan& pp : schedule_passes(merge_single(prims), ns, tq)) compile_pass(st.prog, pp, dense_min);
             steps.push_back(std::move(st));
         }
         if (j < n) {

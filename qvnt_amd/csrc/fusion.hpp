@@ -15,7 +15,11 @@
 //      then commutes with everything it overtakes;
 //   4. split each pass into phases: a phase holds four of the tile's qubits
 //      in registers (16 amplitudes per thread); its matrix primitives act
-//      on those, diagonals on any qubit.
+//      on those, diagonals on any qubit;
+//   5. (dense_min > 0, tiles of >= 2^10 amplitudes) a phase of at least
+//      dense_min primitives that all act on its four register qubits only
+//      (operands and controls) becomes one dense 16x16 unitary, applied as
+//      a complex GEMM on the matrix cores.
 #pragma once
 
 #include <cstdint>
@@ -30,6 +34,7 @@ constexpr int kRegBits = 4;                // register qubits per phase
 constexpr int kRegs = 1 << kRegBits;       // amplitudes per thread
 constexpr int kMaxTile = 12;               // t: at most 2^12 amplitudes per tile
 constexpr int kMinFused = 6;               // below 2^6 amplitudes the plain kernels run
+constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane waves: 2^(t-4) >= 64
 
 // ---------------------------------------------------------- the program
 enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2 };
@@ -56,6 +61,8 @@ struct FPhase {
     uint32_t op_begin, op_end;
     uint32_t rpos[kRegBits];  // local positions of the register bits, ascending
     uint32_t sroff[kRegs];    // swizzled LDS offset of register i (the swizzle is XOR-linear)
+    uint32_t dense;           // 1: the phase is one 16x16 unitary on its register bits (matrix cores)
+    uint32_t ucoef;           // dense: offset (in doubles) of U, row-major complex
 };
 
 struct FPass {
@@ -102,7 +109,7 @@ struct Step {
 // Compiles a validated queue for a buffer of 2^ns amplitudes with tiles of
 // at most tq qubits.  Identity gates (Id, a control beyond the buffer) are
 // dropped.
-void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps);
+void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0);
 
 }  // namespace fusion
 }  // namespace qvnt

@@ -106,6 +106,12 @@ int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg
 
 int qvnt_reg_new(uint32_t q_num, qvnt_prec prec, qvnt_reg** out) { return qvnt_reg_with_state(q_num, 0, prec, out); }
 
+int qvnt_reg_set_dense(qvnt_reg* r, uint32_t min_ops) {
+    if (!r) return fail(QVNT_E_ARG, "null register");
+    r->dense_min = min_ops;
+    return QVNT_OK;
+}
+
 int qvnt_reg_free(qvnt_reg* r) {
     if (!r) return QVNT_OK;
     (void)hipStreamSynchronize(r->stream);

@@ -35,6 +35,7 @@ struct qvnt_reg {
     // fused-form program (phases, micro-ops, coefficients) on the device
     void* prog = nullptr;
     size_t prog_cap = 0;
+    uint32_t dense_min = 0;  // fused form: phases of >= this many ops run as dense 16x16 (0: never)
 
     // per-kernel-class timing
     bool owns_memory = true;  // false: state, workspace and stream belong to a server slot

@@ -104,6 +104,11 @@ class QReg:
         check(lib.qvnt_reg_apply_fused(self._h, op.handle, max_qubits, ctypes.byref(passes)))
         return passes.value
 
+    def set_dense(self, min_ops):
+        """Fused form: fold phases of >= min_ops primitives on four register
+        qubits into dense 16x16 unitaries on the matrix cores (0: never)."""
+        check(lib.qvnt_reg_set_dense(self._h, min_ops))
+
     def apply_gates(self, gates):
         arr = (type(gates[0]) * len(gates))(*gates) if gates else None
         check(lib.qvnt_reg_apply_gates(self._h, arr, len(gates)))

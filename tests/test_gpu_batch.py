@@ -6,7 +6,7 @@ import pytest
 
 import qvnt_amd
 from qvnt_amd import QReg, circuits, op
-from qvnt_amd.batch import TUNE_CAPS, Batch
+from qvnt_amd.batch import TUNE_CAPS, TUNE_DENSE, Batch
 
 pytestmark = pytest.mark.gpu
 
@@ -22,24 +22,28 @@ def infidelity(a, b):
     return 1.0 - abs(np.vdot(a.astype(np.complex128), b.astype(np.complex128))) ** 2
 
 
-@pytest.mark.parametrize("cap", [6, 9, 12])
-def test_batch_equals_single_fused(cap):
-    regs, states, ops = workload(40, 100 + cap)
+@pytest.mark.parametrize("cap,dense", [(6, 0), (9, 0), (12, 0), (12, 2), (10, 3)])
+def test_batch_equals_single_fused(cap, dense):
+    regs, states, ops = workload(40, 100 + cap + dense, n_hi=16 if dense else 14)
     with Batch([n for n, _ in regs], states) as b:
-        info = b.compile(ops, max_qubits=cap)
+        info = b.compile(ops, max_qubits=cap, dense_min=dense)
         b.run()
         assert info["single_ops"] == sum(len(o) for o in ops)
+        if dense:
+            assert info["dense_phases"] > 0
         for k, (n, _) in enumerate(regs):
             with QReg.with_state(n, states[k]) as r:
+                r.set_dense(dense)
                 r.apply_fused(ops[k], max_qubits=cap)
                 want = r.get_state()
             assert np.array_equal(b.get_state(k), want[: 1 << n]), "register %d (%d qubits)" % (k, n)
 
 
-def test_batch_matches_plain_and_oracle(orc):
-    regs, states, ops = workload(24, 7, n_lo=6, n_hi=12)
+@pytest.mark.parametrize("dense", [0, 2])
+def test_batch_matches_plain_and_oracle(orc, dense):
+    regs, states, ops = workload(24, 7 + dense, n_lo=6, n_hi=12)
     with Batch([n for n, _ in regs], states) as b:
-        b.compile(ops, max_qubits=10)
+        b.compile(ops, max_qubits=10, dense_min=dense)
         b.run()
         for k, (n, specs) in enumerate(regs):
             with QReg.with_state(n, states[k]) as r:
@@ -60,6 +64,9 @@ def test_autotune_and_reset():
         for n, cap in info["tile_cap"].items():
             assert cap in TUNE_CAPS
             assert info["tune_ms"][n], "qubit count %d was not timed" % n
+            assert info["dense_min"][n] in (0,) + TUNE_DENSE
+            if min(cap, n) >= 10:
+                assert info["dense_ms"][n], "dense thresholds were not timed for %d qubits" % n
         b.run()
         got = [b.get_state(k) for k in range(len(regs))]
     for k, (n, _) in enumerate(regs):
@@ -96,10 +103,11 @@ def test_graph_replays_match_eager():
         assert np.array_equal(twice, r.get_state()[: 1 << n])
 
 
-def test_fp32_batch_error_bound():
-    regs, states, ops = workload(24, 9, n_lo=8, n_hi=14)
+@pytest.mark.parametrize("dense", [0, 2])
+def test_fp32_batch_error_bound(dense):
+    regs, states, ops = workload(24, 9 + dense, n_lo=8, n_hi=14)
     with Batch([n for n, _ in regs], states, prec="f32") as b:
-        b.compile(ops, max_qubits=12)
+        b.compile(ops, max_qubits=12, dense_min=dense)
         b.run()
         for k, (n, _) in enumerate(regs):
             with QReg.with_state(n, states[k]) as r:

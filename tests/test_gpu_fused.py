@@ -42,6 +42,29 @@ def test_random_circuits(orc, n, tq):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("n,tq,dense,prec", [(10, 10, 2, "f64"), (13, 12, 2, "f64"), (16, 11, 3, "f64"),
+                                             (14, 12, 2, "f32")])
+def test_dense_phases(orc, n, tq, dense, prec):
+    """Phases folded into dense 16x16 unitaries on the matrix cores."""
+    say("dense n=%d tq=%d min_ops=%d %s" % (n, tq, dense, prec))
+    g = np.random.default_rng(7000 + n)
+    specs = random_circuit(g, n, 300) + [s + (False,) for l in range(4) for s in circuits.brickwork_layer(n, l)]
+    ref = orc.Reg(n, threads=8)
+    ref.apply(build(specs, orc.op))
+    with QReg(n, prec=prec) as r:
+        r.set_dense(dense)
+        r.apply_fused(build(specs, op), max_qubits=tq)
+        got = r.get_state().astype(np.complex128)
+    want = ref.state()
+    err = np.max(np.abs(got - want))
+    infid = 1.0 - abs(np.vdot(got, want)) ** 2
+    say("  max |d| = %.2e, infidelity %.2e" % (err, infid))
+    if prec == "f64":
+        assert err <= TOL
+    else:
+        assert err <= 1e-5 and infid <= 1e-5
+
+
 def test_small_registers_take_the_plain_kernels(orc):
     for n in range(0, 6):
         g = np.random.default_rng(n)
