@@ -25,93 +25,115 @@
 namespace qvnt {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // k_init
+constexpr int kMaxBlock = 1024;   // gate kernels: block size chosen at launch (block_setting)
 
 // A dispatch carries its size in work-items as a 32-bit number, so grids
 // are capped at kMaxGrid blocks and every kernel strides over chunks of
-// kBlock * U orbits.
+// blockDim * U orbits.
 constexpr uint64_t kMaxGrid = 1u << 20;
 
-template <int K, class R, int U>
-__global__ __launch_bounds__(kBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+// Loads and stores of one amplitude; NT marks them non-temporal (streamed
+// through the caches with early eviction), used when no other access of
+// the same launch touches the line.
+template <bool NT, class R>
+__device__ __forceinline__ cplx<R> ld(const cplx<R>* p) {
+    if constexpr (NT) return cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+    else return *p;
+}
+template <bool NT, class R>
+__device__ __forceinline__ void st(cplx<R>* p, cplx<R> v) {
+    if constexpr (NT) {
+        __builtin_nontemporal_store(v.re, &p->re);
+        __builtin_nontemporal_store(v.im, &p->im);
+    } else {
+        *p = v;
+    }
+}
+
+template <int K, class R, int U, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+  const uint64_t B = blockDim.x, per = B * U;
+  const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
+    const uint64_t base = chunk * per + threadIdx.x;
     cplx<R> v[U];
     uint64_t idx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
+        const uint64_t t = base + (uint64_t)u * B;
         idx[u] = deposit(t, dep);
-        if (t < groups) v[u] = psi[idx[u]];
+        if (t < groups) v[u] = ld<NT>(psi + idx[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
-        if (t < groups) psi[idx[u]] = f_diag<K>(g, idx[u], v[u]);
+        const uint64_t t = base + (uint64_t)u * B;
+        if (t < groups) st<NT>(psi + idx[u], f_diag<K>(g, idx[u], v[u]));
     }
   }
 }
 
-template <int K, class R, int U>
-__global__ __launch_bounds__(kBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+template <int K, class R, int U, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+  const uint64_t B = blockDim.x, per = B * U;
+  const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
+    const uint64_t base = chunk * per + threadIdx.x;
     cplx<R> v0[U], v1[U];
     uint64_t i0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
+        const uint64_t t = base + (uint64_t)u * B;
         i0[u] = deposit(t, dep);
         if (t < groups) {
-            v0[u] = psi[i0[u]];
-            v1[u] = psi[i0[u] ^ g.a];
+            v0[u] = ld<NT>(psi + i0[u]);
+            v1[u] = ld<NT>(psi + (i0[u] ^ g.a));
         }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
+        const uint64_t t = base + (uint64_t)u * B;
         if (t < groups) {
             const uint64_t i1 = i0[u] ^ g.a;
             const cplx<R> o0 = f_pair<K>(g, i0[u], v0[u], v1[u]);
             const cplx<R> o1 = f_pair<K>(g, i1, v1[u], v0[u]);
-            psi[i0[u]] = o0;
-            psi[i1] = o1;
+            st<NT>(psi + i0[u], o0);
+            st<NT>(psi + i1, o1);
         }
     }
   }
 }
 
-template <int K, class R, int U>
-__global__ __launch_bounds__(kBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
-  const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+template <int K, class R, int U, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+  const uint64_t B = blockDim.x, per = B * U;
+  const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint64_t base = chunk * (kBlock * U) + threadIdx.x;
+    const uint64_t base = chunk * per + threadIdx.x;
     cplx<R> v[U][4];
     uint64_t i0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
+        const uint64_t t = base + (uint64_t)u * B;
         i0[u] = deposit(t, dep);
         if (t < groups) {
-            v[u][0] = psi[i0[u]];
-            v[u][1] = psi[i0[u] | g.a];
-            v[u][2] = psi[i0[u] | g.b];
-            v[u][3] = psi[i0[u] | g.a | g.b];
+            v[u][0] = ld<NT>(psi + i0[u]);
+            v[u][1] = ld<NT>(psi + (i0[u] | g.a));
+            v[u][2] = ld<NT>(psi + (i0[u] | g.b));
+            v[u][3] = ld<NT>(psi + (i0[u] | g.a | g.b));
         }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * kBlock;
+        const uint64_t t = base + (uint64_t)u * B;
         if (t < groups) {
             cplx<R> o[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) o[s] = f_quad<K>(g, s, v[u]);
-            psi[i0[u]] = o[0];
-            psi[i0[u] | g.a] = o[1];
-            psi[i0[u] | g.b] = o[2];
-            psi[i0[u] | g.a | g.b] = o[3];
+            st<NT>(psi + i0[u], o[0]);
+            st<NT>(psi + (i0[u] | g.a), o[1]);
+            st<NT>(psi + (i0[u] | g.b), o[2]);
+            st<NT>(psi + (i0[u] | g.a | g.b), o[3]);
         }
     }
   }
@@ -125,13 +147,23 @@ __global__ __launch_bounds__(kBlock) void k_init(cplx<R>* __restrict__ psi, uint
 }
 
 // ----------------------------------------------------------- host side
+// Orbits per thread and threads per block of the gate kernels (tuning
+// knobs; the defaults are the measured best on a 2^30-amplitude state).
 int unroll_setting() {
     static int u = [] {
         const char* s = std::getenv("QVNT_UNROLL");
         int v = s ? std::atoi(s) : 2;
-        return (v == 1 || v == 2 || v == 4) ? v : 2;
+        return (v == 1 || v == 2) ? v : 2;
     }();
     return u;
+}
+unsigned block_setting() {
+    static unsigned b = [] {
+        const char* s = std::getenv("QVNT_BLOCK");
+        int v = s ? std::atoi(s) : 256;
+        return (unsigned)((v == 256 || v == 512 || v == 1024) ? v : 256);
+    }();
+    return b;
 }
 
 void add_pos(Deposit& d, unsigned p) {
@@ -160,17 +192,18 @@ Par<R> make_par(const qvnt_gate& g) {
     return p;
 }
 
-template <class R, int U>
+template <class R, int U, bool NT>
 int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
     const Par<R> p = make_par<R>(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
-    const uint64_t nchunks = (groups + (uint64_t)kBlock * U - 1) / ((uint64_t)kBlock * U);
+    const unsigned B = block_setting();
+    const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
     const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid));
-    const dim3 block(kBlock);
+    const dim3 block(B);
     hipStream_t s = r->stream;
 #define QVNT_CASE(KIND, KERNEL)                                            \
     case KIND:                                                             \
-        hipLaunchKernelGGL((KERNEL<KIND, R, U>), grid, block, 0, s, psi, groups, d, p); \
+        hipLaunchKernelGGL((KERNEL<KIND, R, U, NT>), grid, block, 0, s, psi, groups, d, p); \
         break;
     if (cls == GClass::Diag) {
         switch (gate.kind) {
@@ -210,13 +243,17 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
     return QVNT_OK;
 }
 
+// Non-temporal accesses when every amplitude's 128-byte line is touched by
+// one load and one store only: always for diagonals; for pairs and quads
+// when the partner offsets are at least 8 amplitudes (the partners lie in
+// other lines, read by other instructions).
 template <class R>
 int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
-    switch (unroll_setting()) {
-    case 1: return launch_kernel<R, 1>(r, gate, cls, groups, d);
-    case 4: return launch_kernel<R, 4>(r, gate, cls, groups, d);
-    default: return launch_kernel<R, 2>(r, gate, cls, groups, d);
-    }
+    const uint64_t partner = cls == GClass::Diag ? 0 : cls == GClass::Pair ? gate.a : (gate.a | gate.b);
+    const bool nt = cls == GClass::Diag || (partner & 7u) == 0;
+    if (unroll_setting() == 1)
+        return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d) : launch_kernel<R, 1, false>(r, gate, cls, groups, d);
+    return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d) : launch_kernel<R, 2, false>(r, gate, cls, groups, d);
 }
 
 }  // namespace
