@@ -293,6 +293,67 @@ int qvnt_reg_set_dense(qvnt_reg* reg, uint32_t min_ops);
 int qvnt_reg_kernel_stats(qvnt_reg* reg, qvnt_kstats* out, int reset);
 
 /* ------------------------------------------------------------------ */
+/* Reset, register algebra and readback (src/register/quant.rs). */
+/* QReg::reset (quant.rs:202-205): zeros, psi[i_state & q_mask] = 1. */
+int qvnt_reg_reset(qvnt_reg* reg, uint64_t i_state);
+/* QReg::reset_by_mask (quant.rs:207-229): the whole register -> reset(0);
+ * otherwise every amplitude with idx & mask != 0 is zeroed, then
+ * QReg::normalize (quant.rs:397-414): norm = sqrt(sum |psi|^2); reset(0) if
+ * norm <= 1e-15, unchanged if 1 - norm <= 1e-9, else scaled by 1/norm. */
+int qvnt_reg_reset_by_mask(qvnt_reg* reg, uint64_t mask);
+/* Mul for QReg (tensor_prod, quant.rs:330-371, 625-636): a new register of
+ * a.q_num + b.q_num qubits, a on the low qubits.  a and b are kept. */
+int qvnt_reg_tensor(qvnt_reg* a, qvnt_reg* b, qvnt_reg** out);
+/* QReg::combine (quant.rs:245-268): equal sizes (>= 3 qubits; below, the
+ * reference panics as its 8-element buffers do not fit as halves); a fills
+ * the lower half of a register one qubit larger, b the upper half. */
+int qvnt_reg_combine(qvnt_reg* a, qvnt_reg* b, qvnt_reg** out);
+/* QReg::combine_with_unitary (quant.rs:271-308): m = [c00, c01, c10, c11]
+ * (interleaved re, im), unitary by the reference's rule. */
+int qvnt_reg_combine_with_unitary(qvnt_reg* a, qvnt_reg* b, const double m[8], qvnt_reg** out);
+/* QReg::linear_composition (quant.rs:311-328): reg = reg * c0 + other * c1,
+ * c = [c0.re, c0.im, c1.re, c1.im]; equal buffer lengths. */
+int qvnt_reg_linear_composition(qvnt_reg* reg, qvnt_reg* other, const double c[4]);
+/* QReg::set_num (quant.rs:156-166): the state is zero-extended or
+ * truncated to 2^q_num amplitudes; nothing is renormalised (the reference's
+ * reset branch is dead code). */
+int qvnt_reg_set_num(qvnt_reg* reg, uint32_t q_num);
+/* QReg::get_polar (quant.rs:417-431): (|psi_i|, arg psi_i) pairs, FP64,
+ * for i < 2^n; dst holds 2 * 2^n doubles. */
+int qvnt_reg_get_polar(qvnt_reg* reg, double* dst, uint64_t len);
+/* QReg::sample_all (quant.rs:513-594) as an exact multinomial: `shots`
+ * draws qvnt_uniform(seed, stream, k), k = 0..shots-1, under the measurement
+ * contract, without collapse; hist[i] = count of outcome i, one entry per
+ * buffer element (len >= max(2^n, 8)).  Replaces the reference's normal
+ * approximation on an unseeded generator. */
+int qvnt_reg_sample_all(qvnt_reg* reg, uint64_t shots, uint64_t seed, uint64_t stream, uint64_t* hist, uint64_t len);
+
+/* ------------------------------------------------------------------ */
+/* The QASM driver loop: Sym::finish (src/qasm/sym.rs:40-74) over the
+ * (op, separator) list the interpreter builds (qasm/int/ext_op.rs:5-16). */
+typedef enum {
+    QVNT_SEP_NOP = 0,      /* apply op */
+    QVNT_SEP_MEASURE = 1,  /* apply op; measure_mask(a) -> classical bits b (Sep::Measure(q_arg, c_arg)) */
+    QVNT_SEP_IF = 2,       /* apply op only if the classical bits a read b (Sep::IfBranch(c, v)) */
+    QVNT_SEP_RESET = 3     /* apply op; reset_by_mask(a) (Sep::Reset(q)) */
+} qvnt_sep;
+typedef struct {
+    const qvnt_op* op;     /* may be null: no gates */
+    uint32_t sep;          /* qvnt_sep */
+    uint32_t pad;
+    uint64_t a, b;
+} qvnt_step;
+/* Runs the steps, then `tail` (may be null), on reg and the classical
+ * value *c_value of c_num bits.  Measurement: the i-th set bit of the
+ * classical mask is set from (or, measure_xor, xored with) the i-th set bit
+ * of the measured mask (MeasureOp::Set / Xor); each measurement with a
+ * non-empty mask takes the next draw; *used = draws taken.  If: the
+ * classical bits at the set bits of a, packed from bit 0 (CReg::get_by_mask,
+ * class.rs:106-117), must equal b. */
+int qvnt_reg_run_program(qvnt_reg* reg, const qvnt_step* steps, size_t n, const qvnt_op* tail, uint32_t measure_xor,
+                         uint64_t* c_value, uint32_t c_num, const double* draws, size_t n_draws, size_t* used);
+
+/* ------------------------------------------------------------------ */
 /* Batched uneven registers (no counterpart in the reference, which runs
  * one register at a time): many independent registers of 6..30 qubits in
  * one device arena, each with its own circuit, compiled into fused passes

@@ -40,6 +40,14 @@ class Gate(ctypes.Structure):
 
 assert ctypes.sizeof(Gate) == 312
 
+SEP_NOP, SEP_MEASURE, SEP_IF, SEP_RESET = 0, 1, 2, 3
+
+
+class Step(ctypes.Structure):
+    """Mirror of ``orc_step``: one (op, separator) entry of qasm's ExtOp."""
+    _fields_ = [("g", ctypes.POINTER(Gate)), ("n", ctypes.c_size_t), ("sep", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32), ("a", ctypes.c_uint64), ("b", ctypes.c_uint64)]
+
 
 def build(force=False):
     """Compile the oracle with its Makefile (gcc, no GPU needed)."""
@@ -90,6 +98,21 @@ def lib():
         L.orc_tree_sample.restype = u64
         L.orc_splitmix64.argtypes, L.orc_splitmix64.restype = [ctypes.POINTER(u64)], u64
         L.orc_uniform.argtypes, L.orc_uniform.restype = [u64, u64, u64], d
+        vp, pd = ctypes.c_void_p, ctypes.POINTER(d)
+        L.orc_reg_reset.argtypes, L.orc_reg_reset.restype = [vp, u64], None
+        L.orc_reg_reset_by_mask.argtypes, L.orc_reg_reset_by_mask.restype = [vp, u64], None
+        L.orc_reg_tensor.argtypes, L.orc_reg_tensor.restype = [vp, vp], vp
+        L.orc_reg_combine.argtypes, L.orc_reg_combine.restype = [vp, vp], vp
+        L.orc_reg_combine_with_unitary.argtypes, L.orc_reg_combine_with_unitary.restype = [vp, vp, pd], vp
+        L.orc_reg_linear_composition.argtypes, L.orc_reg_linear_composition.restype = [vp, vp, pd], ctypes.c_int
+        L.orc_reg_set_num.argtypes, L.orc_reg_set_num.restype = [vp, ctypes.c_uint32], ctypes.c_int
+        L.orc_reg_get_polar.argtypes, L.orc_reg_get_polar.restype = [vp, pd], None
+        L.orc_reg_sample_all.argtypes = [vp, u64, u64, u64, ctypes.POINTER(u64)]
+        L.orc_reg_sample_all.restype = ctypes.c_int
+        L.orc_run_program.argtypes = [vp, ctypes.POINTER(Step), ctypes.c_size_t, gp, ctypes.c_size_t, ctypes.c_uint32,
+                                      ctypes.POINTER(u64), ctypes.c_uint32, pd, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_size_t)]
+        L.orc_run_program.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -201,6 +224,73 @@ class Reg:
         if err.value:
             raise ZeroDivisionError("measure on a zero-norm state")
         return r
+
+    @classmethod
+    def _wrap(cls, h, q_num):
+        if not h:
+            return None
+        r = cls.__new__(cls)
+        r.h, r.q_num = h, q_num
+        return r
+
+    def reset(self, i_state=0):
+        lib().orc_reg_reset(self.h, i_state)
+
+    def reset_by_mask(self, mask):
+        lib().orc_reg_reset_by_mask(self.h, mask)
+
+    def __mul__(self, other):
+        """Tensor product: self on the low qubits (quant.rs:330-371)."""
+        return Reg._wrap(lib().orc_reg_tensor(self.h, other.h), self.q_num + other.q_num)
+
+    def combine(self, other):
+        return Reg._wrap(lib().orc_reg_combine(self.h, other.h), self.q_num + 1)
+
+    def combine_with_unitary(self, other, m):
+        c = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(4).view(np.float64))
+        return Reg._wrap(lib().orc_reg_combine_with_unitary(self.h, other.h,
+                                                            c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                         self.q_num + 1)
+
+    def linear_composition(self, other, c0, c1):
+        c = np.array([c0.real, c0.imag, c1.real, c1.imag], np.float64)
+        if lib().orc_reg_linear_composition(self.h, other.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise ValueError("buffers of different length")
+
+    def set_num(self, q_num):
+        if lib().orc_reg_set_num(self.h, q_num):
+            raise MemoryError("set_num")
+        self.q_num = q_num
+
+    def get_polar(self):
+        out = np.empty(2 << self.q_num, np.float64)
+        lib().orc_reg_get_polar(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out.reshape(-1, 2)
+
+    def sample_all(self, shots, seed, stream):
+        out = np.zeros(len(self), np.uint64)
+        if lib().orc_reg_sample_all(self.h, shots, seed, stream, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))):
+            raise ZeroDivisionError("zero-norm state")
+        return out
+
+    def run_program(self, steps, tail, c_num, c_value=0, measure_xor=False, draws=()):
+        """Sym::finish (qasm/sym.rs:40-74).  steps: (Op, sep, a, b) tuples;
+        returns (classical value, draws used)."""
+        keep = [st[0]._arr() for st in steps]
+        arr = (Step * max(1, len(steps)))()
+        for k, (o, sep, a, b) in enumerate(steps):
+            arr[k].g = ctypes.cast(keep[k], ctypes.POINTER(Gate))
+            arr[k].n = len(o.gates)
+            arr[k].sep, arr[k].a, arr[k].b = sep, a, b
+        d = np.ascontiguousarray(draws, np.float64)
+        cval = ctypes.c_uint64(c_value)
+        used = ctypes.c_size_t(0)
+        rc = lib().orc_run_program(self.h, arr, len(steps), tail._arr(), len(tail.gates), 1 if measure_xor else 0,
+                                   ctypes.byref(cval), c_num, d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   len(d), ctypes.byref(used))
+        if rc:
+            raise RuntimeError("program failed with status %d" % rc)
+        return cval.value, used.value
 
 
 def tree_total(psi, count=None):

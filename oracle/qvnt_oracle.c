@@ -659,3 +659,204 @@ double orc_uniform(uint64_t seed, uint64_t stream, uint64_t index) {
     uint64_t z = key + (index + 1u) * 0x9E3779B97F4A7C15ull;
     return (double)(mix64(z) >> 11) * (1.0 / 9007199254740992.0);
 }
+
+/* ------------------------------------------- register algebra and reset */
+/* QReg::reset (quant.rs:202-205): zeros, psi[q_mask & i_state] = 1. */
+void orc_reg_reset(orc_reg* r, uint64_t i_state) {
+    memset(r->psi, 0, r->len * sizeof(orc_c));
+    r->psi[r->q_mask & i_state].re = 1.0;
+}
+
+/* QReg::normalize (quant.rs:397-414): norm = sqrt(get_absolute()); reset
+ * for norm <= 1e-15; untouched when 1 - norm <= 1e-9 (also for norm > 1);
+ * otherwise every amplitude times 1/norm. */
+static void normalize(orc_reg* r) {
+    double norm = sqrt(orc_reg_get_absolute(r));
+    uint64_t i;
+    if (norm <= 1e-15) { orc_reg_reset(r, 0); return; }
+    if (1. - norm <= 1e-9) return;
+    norm = 1. / norm;
+    for (i = 0; i < r->len; ++i) r->psi[i] = c_scale(r->psi[i], norm);
+}
+
+/* QReg::reset_by_mask (quant.rs:207-229): the whole register -> reset(0);
+ * else zero every amplitude with idx & mask != 0, then normalize. */
+void orc_reg_reset_by_mask(orc_reg* r, uint64_t mask) {
+    uint64_t i;
+    if ((mask & r->q_mask) == r->q_mask) { orc_reg_reset(r, 0); return; }
+    for (i = 0; i < r->len; ++i)
+        if (i & mask) { r->psi[i].re = 0.0; r->psi[i].im = 0.0; }
+    normalize(r);
+}
+
+static orc_reg* alloc_reg(uint32_t q_num, uint64_t len) {
+    orc_reg* r = (orc_reg*)calloc(1, sizeof(orc_reg));
+    if (!r) return NULL;
+    r->q_num = q_num;
+    r->q_mask = (1ull << q_num) - 1u;
+    r->len = len;
+    r->threads = 1;
+    r->psi = (orc_c*)calloc(len, sizeof(orc_c));
+    if (!r->psi) { free(r); return NULL; }
+    return r;
+}
+
+/* tensor_prod (quant.rs:330-371): q_num = a + b; psi[idx] =
+ * a[idx & a_mask] * b[(idx >> a.q_num) & b_mask] for idx < 2^q_num, zero
+ * in the padding of the max(2^q_num, 8) buffer. */
+orc_reg* orc_reg_tensor(const orc_reg* a, const orc_reg* b) {
+    uint32_t q_num = a->q_num + b->q_num;
+    uint64_t q_size = 1ull << q_num, len = q_size > MIN_BUFFER_LEN ? q_size : MIN_BUFFER_LEN, idx;
+    orc_reg* r = alloc_reg(q_num, len);
+    if (!r) return NULL;
+    for (idx = 0; idx < q_size; ++idx)
+        r->psi[idx] = c_mul(a->psi[idx & a->q_mask], b->psi[(idx >> a->q_num) & b->q_mask]);
+    return r;
+}
+
+/* QReg::combine (quant.rs:245-268): equal sizes; a fills the lower half
+ * of a register one qubit larger, b the upper half.  NULL where the
+ * reference returns None (sizes differ) or panics (the halves do not fit:
+ * fewer than 3 qubits, where both buffers hold the 8-element minimum). */
+orc_reg* orc_reg_combine(const orc_reg* a, const orc_reg* b) {
+    uint64_t q_size, len;
+    orc_reg* r;
+    if (a->q_num != b->q_num || a->len != b->len) return NULL;
+    q_size = 1ull << (a->q_num + 1);
+    len = q_size > MIN_BUFFER_LEN ? q_size : MIN_BUFFER_LEN;
+    if (len != 2 * a->len) return NULL;
+    r = alloc_reg(a->q_num + 1, len);
+    if (!r) return NULL;
+    memcpy(r->psi, a->psi, a->len * sizeof(orc_c));
+    memcpy(r->psi + a->len, b->psi, b->len * sizeof(orc_c));
+    return r;
+}
+
+/* QReg::combine_with_unitary (quant.rs:271-308): psi[idx] = c00 a[m & idx]
+ * + c01 b[m & idx] where idx has no bit above m (m = a.q_mask), else
+ * c10 a[..] + c11 b[..]; c = [c00, c01, c10, c11] interleaved. */
+orc_reg* orc_reg_combine_with_unitary(const orc_reg* a, const orc_reg* b, const double c[8]) {
+    uint64_t q_size, len, idx, m = a->q_mask;
+    orc_reg* r;
+    orc_c c00 = {c[0], c[1]}, c01 = {c[2], c[3]}, c10 = {c[4], c[5]}, c11 = {c[6], c[7]};
+    if (a->q_num != b->q_num) return NULL;
+    q_size = 1ull << (a->q_num + 1);
+    len = q_size > MIN_BUFFER_LEN ? q_size : MIN_BUFFER_LEN;
+    r = alloc_reg(a->q_num + 1, len);
+    if (!r) return NULL;
+    for (idx = 0; idx < len; ++idx) {
+        orc_c q0 = a->psi[m & idx], q1 = b->psi[m & idx];
+        r->psi[idx] = (~m & idx) == 0 ? c_add(c_mul(c00, q0), c_mul(c01, q1)) : c_add(c_mul(c10, q0), c_mul(c11, q1));
+    }
+    return r;
+}
+
+/* QReg::linear_composition (quant.rs:311-328): psi = psi * c0 + other * c1
+ * element by element over buffers of equal length; -1 if they differ. */
+int orc_reg_linear_composition(orc_reg* r, const orc_reg* other, const double c[4]) {
+    orc_c c0 = {c[0], c[1]}, c1 = {c[2], c[3]};
+    uint64_t i;
+    if (r->len != other->len) return -1;
+    for (i = 0; i < r->len; ++i) r->psi[i] = c_add(c_mul(r->psi[i], c0), c_mul(other->psi[i], c1));
+    return 0;
+}
+
+/* QReg::set_num (quant.rs:156-166): the buffer is resized to exactly 2^q
+ * (zero-extended or truncated); the reset branch after it is dead (it
+ * compares with the already-updated q_num), so nothing is renormalised. */
+int orc_reg_set_num(orc_reg* r, uint32_t q_num) {
+    uint64_t q_size = 1ull << q_num;
+    orc_c* psi = (orc_c*)calloc(q_size, sizeof(orc_c));
+    if (!psi) return -1;
+    memcpy(psi, r->psi, (q_size < r->len ? q_size : r->len) * sizeof(orc_c));
+    free(r->psi);
+    r->psi = psi;
+    r->len = q_size;
+    r->q_num = q_num;
+    r->q_mask = q_size - 1u;
+    return 0;
+}
+
+/* QReg::get_polar (quant.rs:417-431): num-complex to_polar = (norm, arg) =
+ * (hypot(re, im), atan2(im, re)) for the first 2^q amplitudes. */
+void orc_reg_get_polar(const orc_reg* r, double* out) {
+    uint64_t i, n = 1ull << r->q_num;
+    for (i = 0; i < n; ++i) {
+        out[2 * i] = hypot(r->psi[i].re, r->psi[i].im);
+        out[2 * i + 1] = atan2(r->psi[i].im, r->psi[i].re);
+    }
+}
+
+/* sample_all (quant.rs:513-594) as an exact multinomial: `shots` seeded
+ * draws u_k = orc_uniform(seed, stream, k) through the tree contract over
+ * the 2^q weights; hist has one count per buffer element.  The reference's
+ * normal approximation with an unseeded generator is not reproducible. */
+int orc_reg_sample_all(const orc_reg* r, uint64_t shots, uint64_t seed, uint64_t stream, uint64_t* hist) {
+    tree_t t;
+    uint64_t k;
+    int err = 0;
+    memset(hist, 0, r->len * sizeof(uint64_t));
+    if (shots == 0) return 0;
+    tree_build(&t, r->psi, 1ull << r->q_num);
+    for (k = 0; k < shots && !err; ++k) {
+        uint64_t j = tree_descend(&t, r->psi, orc_uniform(seed, stream, k), &err);
+        if (!err) hist[j] += 1;
+    }
+    tree_free(&t);
+    return err;
+}
+
+/* ------------------------------------------------ the QASM driver loop */
+/* CReg::get_by_mask (class.rs:106-117): the value's bits at the set bits of
+ * mask & q_mask, packed from bit 0 upwards. */
+static uint64_t get_by_mask(uint64_t value, uint64_t mask) {
+    uint64_t out = 0, m;
+    unsigned k = 0;
+    for (m = mask; m; m &= m - 1, ++k)
+        if (value & (m & (~m + 1))) out |= 1ull << k;
+    return out;
+}
+
+/* Sym::finish (qasm/sym.rs:40-74) on a register and a classical value of
+ * c_num bits: for each step apply its op, then by separator
+ *   MEASURE(a = q_arg, b = c_arg): measure_mask(q_arg) with the next draw
+ *     (none is taken when q_arg & q_mask == 0); the i-th set bit of c_arg
+ *     is set (or xored, measure_xor) from the i-th set bit of q_arg;
+ *   IF(a = c, b = v): the op runs only if get_by_mask(c) == v;
+ *   RESET(a = q): reset_by_mask(q);
+ * and finally the tail op. */
+int orc_run_program(orc_reg* r, const orc_step* steps, size_t n, const qvnt_gate* tail, size_t tail_n,
+                    uint32_t measure_xor, uint64_t* cval, uint32_t c_num, const double* draws, size_t n_draws,
+                    size_t* used) {
+    uint64_t c_mask = c_num >= 64 ? ~0ull : ((1ull << c_num) - 1u);
+    size_t k;
+    *used = 0;
+    for (k = 0; k < n; ++k) {
+        const orc_step* s = &steps[k];
+        if (s->sep == ORC_SEP_IF) {
+            if (get_by_mask(*cval, s->a & c_mask) == s->b) orc_reg_apply(r, s->g, s->n);
+            continue;
+        }
+        orc_reg_apply(r, s->g, s->n);
+        if (s->sep == ORC_SEP_MEASURE) {
+            uint64_t outcome = 0, qm, cm;
+            if (s->a & r->q_mask) {
+                int err = 0;
+                if (*used >= n_draws) return QVNT_E_ARG;
+                outcome = orc_reg_measure_mask(r, s->a, draws[(*used)++], 0, &err);
+                if (err) return err;
+            }
+            for (qm = s->a, cm = s->b; qm && cm; qm &= qm - 1, cm &= cm - 1) {
+                uint64_t qb = qm & (~qm + 1), cb = cm & (~cm + 1);
+                int bit = (outcome & qb) != 0;
+                if (measure_xor) { if (bit) *cval ^= cb; }
+                else if (bit) *cval |= cb;
+                else *cval &= ~cb;
+            }
+        } else if (s->sep == ORC_SEP_RESET) {
+            orc_reg_reset_by_mask(r, s->a);
+        }
+    }
+    orc_reg_apply(r, tail, tail_n);
+    return 0;
+}

@@ -99,6 +99,31 @@ uint64_t orc_reg_measure_mask(orc_reg* r, uint64_t mask, double u, uint32_t flag
 double   orc_tree_total(const double* psi_interleaved, uint64_t count);
 uint64_t orc_tree_sample(const double* psi_interleaved, uint64_t count, double u, int* err);
 
+/* ---- reset, register algebra, readback (quant.rs:156-166, 202-229,
+ * 245-371, 397-431, 513-594) ---- */
+void     orc_reg_reset(orc_reg* r, uint64_t i_state);
+void     orc_reg_reset_by_mask(orc_reg* r, uint64_t mask);
+orc_reg* orc_reg_tensor(const orc_reg* a, const orc_reg* b);
+orc_reg* orc_reg_combine(const orc_reg* a, const orc_reg* b);
+orc_reg* orc_reg_combine_with_unitary(const orc_reg* a, const orc_reg* b, const double c[8]);
+int      orc_reg_linear_composition(orc_reg* r, const orc_reg* other, const double c[4]);
+int      orc_reg_set_num(orc_reg* r, uint32_t q_num);
+void     orc_reg_get_polar(const orc_reg* r, double* out);
+int      orc_reg_sample_all(const orc_reg* r, uint64_t shots, uint64_t seed, uint64_t stream, uint64_t* hist);
+
+/* ---- Sym::finish (qasm/sym.rs:40-74) ---- */
+enum { ORC_SEP_NOP = 0, ORC_SEP_MEASURE = 1, ORC_SEP_IF = 2, ORC_SEP_RESET = 3 };
+typedef struct orc_step {
+    const qvnt_gate* g;   /* the step's op (queue of SingleOps) */
+    size_t n;
+    uint32_t sep;         /* ORC_SEP_* */
+    uint32_t pad;
+    uint64_t a, b;        /* MEASURE: q_arg, c_arg; IF: c mask, value; RESET: q mask */
+} orc_step;
+int orc_run_program(orc_reg* r, const orc_step* steps, size_t n, const qvnt_gate* tail, size_t tail_n,
+                    uint32_t measure_xor, uint64_t* cval, uint32_t c_num, const double* draws, size_t n_draws,
+                    size_t* used);
+
 /* SplitMix64-based seeded uniform (same definition as qvnt_uniform). */
 uint64_t orc_splitmix64(uint64_t* state);
 double   orc_uniform(uint64_t seed, uint64_t stream, uint64_t index);

@@ -6,7 +6,8 @@ The package loads ``qvnt_amd/lib/libqvnt_hip.so``; there is no CPU path.
 """
 from ._lib import QvntError, Gate, KIND, KINDS, lib, last_error
 from .ops import MultiOp, from_gates, op
-from .register import CReg, QReg, uniform
+from .register import CReg, QReg, VReg, uniform
+from .program import Program, Sym
 
 __all__ = ["QvntError", "Gate", "KIND", "KINDS", "MultiOp", "from_gates", "op",
-           "CReg", "QReg", "uniform", "lib", "last_error"]
+           "CReg", "QReg", "VReg", "Program", "Sym", "uniform", "lib", "last_error"]

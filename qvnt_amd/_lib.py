@@ -99,6 +99,16 @@ def _declare(L):
         "qvnt_reg_set_timing": ([vp, i], i),
         "qvnt_reg_set_dense": ([vp, u32], i),
         "qvnt_reg_kernel_stats": ([vp, ctypes.POINTER(KStats), i], i),
+        "qvnt_reg_reset": ([vp, u64], i),
+        "qvnt_reg_reset_by_mask": ([vp, u64], i),
+        "qvnt_reg_tensor": ([vp, vp, opp], i),
+        "qvnt_reg_combine": ([vp, vp, opp], i),
+        "qvnt_reg_combine_with_unitary": ([vp, vp, dp, opp], i),
+        "qvnt_reg_linear_composition": ([vp, vp, dp], i),
+        "qvnt_reg_set_num": ([vp, u32], i),
+        "qvnt_reg_get_polar": ([vp, dp, u64], i),
+        "qvnt_reg_sample_all": ([vp, u64, u64, u64, u64p, u64], i),
+        "qvnt_reg_run_program": ([vp, ctypes.POINTER(Step), sz, vp, u32, u64p, u32, dp, sz, ctypes.POINTER(sz)], i),
     }
     for name in ["x", "y", "z", "s", "t", "swap", "sqrt_swap", "i_swap", "sqrt_i_swap", "qft", "qft_swapped"]:
         sig["qvnt_op_" + name] = ([u64, opp], i)
@@ -108,6 +118,15 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes, f.restype = args, res
     return L
+
+
+SEP_NOP, SEP_MEASURE, SEP_IF, SEP_RESET = 0, 1, 2, 3
+
+
+class Step(ctypes.Structure):
+    """Mirror of ``qvnt_step`` (one (op, separator) entry of the QASM driver)."""
+    _fields_ = [("op", ctypes.c_void_p), ("sep", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("a", ctypes.c_uint64), ("b", ctypes.c_uint64)]
 
 
 def load():

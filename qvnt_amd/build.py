@@ -16,7 +16,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libqvnt_hip.so")
 
-SOURCES = ["ops.cpp", "fusion.cpp", "plain.hip", "measure.hip", "fused.hip", "reg.hip", "serve.hip", "batch.hip"]
+SOURCES = ["ops.cpp", "fusion.cpp", "plain.hip", "measure.hip", "fused.hip", "reg.hip", "serve.hip", "batch.hip", "algebra.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

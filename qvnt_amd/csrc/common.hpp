@@ -35,5 +35,7 @@ inline uint64_t touched_bits(const qvnt_gate& g) {
 }
 
 bool is_diagonal_kind(uint32_t kind);
+// the reference's unitarity rule for a 2x2 complex matrix (interleaved re, im)
+bool is_unitary_m1(const double* m);
 
 }  // namespace qvnt

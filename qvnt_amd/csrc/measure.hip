@@ -371,6 +371,19 @@ int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm) {
     return QVNT_OK;
 }
 
+int rescale(qvnt_reg* r, double scale) {
+    const Deposit d{};  // no fixed bits: every amplitude of the buffer
+    const uint64_t nb = (r->len + kBlock - 1) / kBlock;
+    const dim3 grid((unsigned)(nb < kMaxGrid ? nb : kMaxGrid));
+    Timed t(r, QVNT_K_COLLAPSE, 2.0 * (double)r->amp_bytes() * (double)r->len);
+    if (r->prec == QVNT_F64)
+        hipLaunchKernelGGL(k_rescale<double>, grid, dim3(kBlock), 0, r->stream, static_cast<c64*>(r->psi), r->len, d, scale);
+    else
+        hipLaunchKernelGGL(k_rescale<float>, grid, dim3(kBlock), 0, r->stream, static_cast<c32*>(r->psi), r->len, d, scale);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n) {
     // bounded device staging: chunks of up to 2^24 probabilities
     const uint64_t chunk = n < (1ull << 24) ? n : (1ull << 24);

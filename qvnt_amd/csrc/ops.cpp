@@ -301,6 +301,9 @@ std::string gate_name(const qvnt_gate& g) {
 }
 
 }  // namespace
+
+bool is_unitary_m1(const double* m) { return unitary_1(m); }
+
 }  // namespace qvnt
 
 using namespace qvnt;

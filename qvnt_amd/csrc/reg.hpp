@@ -78,6 +78,8 @@ size_t tree_doubles(uint64_t count);
 int sample_launch(qvnt_reg* r, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev, double* total_dev);
 int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm);
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n);
+// every amplitude of the buffer times `scale` (FP64 multiply)
+int rescale(qvnt_reg* r, double scale);
 int compare(qvnt_reg* a, qvnt_reg* b, double* out);
 
 // fused.hip

@@ -22,22 +22,104 @@ def uniform(seed, stream, index):
     return lib.qvnt_uniform(seed, stream, index)
 
 
+def _bits(mask):
+    """The set bits of mask as single-bit masks, ascending (math/bits_iter.rs)."""
+    out = []
+    while mask:
+        low = mask & -mask
+        out.append(low)
+        mask ^= low
+    return out
+
+
 class CReg:
-    """Classical result of a measurement (src/register/class.rs:8-12, 66-74)."""
+    """Classical register (src/register/class.rs): a value of q_num bits."""
 
     def __init__(self, q_num, value=0):
         self.q_num = q_num
         self.q_mask = (1 << q_num) - 1 if q_num < 64 else (1 << 64) - 1
         self.value = value
 
+    @classmethod
+    def new(cls, q_num):
+        return cls(q_num, 0)
+
+    @classmethod
+    def with_state(cls, q_num, state):
+        return cls(q_num, state)
+
     def get(self):
         return self.value
+
+    def num(self):
+        return self.q_num
+
+    def set_num(self, q_num):
+        """class.rs:75-78: the value is kept as it is."""
+        self.q_num = q_num
+        self.q_mask = (1 << q_num) - 1 if q_num < 64 else (1 << 64) - 1
+
+    def reset(self, i_state=0):
+        self.value = i_state & self.q_mask
+
+    def set(self, bit, mask):
+        """class.rs:84-90: the bits of mask := bit."""
+        self.value = (self.value | mask) if bit else (self.value & ~mask)
+
+    def xor(self, bit, mask):
+        if bit:
+            self.value ^= mask
+
+    def get_by_mask(self, mask):
+        """class.rs:106-117: the value's bits at the set bits of mask & q_mask, packed from bit 0."""
+        return sum(1 << k for k, b in enumerate(_bits(mask & self.q_mask)) if self.value & b)
+
+    def __mul__(self, other):
+        """Tensor product (class.rs:92-96): self on the low bits."""
+        return CReg(self.q_num + other.q_num, self.value | (other.value << self.q_num))
 
     def __repr__(self):
         return "(" + "".join("1" if self.value >> k & 1 else "0" for k in reversed(range(self.q_num))) + ")"
 
     def __eq__(self, other):
         return isinstance(other, CReg) and (self.q_num, self.value) == (other.q_num, other.value)
+
+
+class VReg:
+    """Virtual register (src/register/virtl.rs): a list of single-bit masks,
+    indexed by position, by a predicate on positions, by a list of
+    positions, or whole ([:]), giving the OR of the selected masks."""
+
+    def __init__(self, num):
+        self.masks = _bits((1 << num) - 1)
+
+    @classmethod
+    def from_mask(cls, mask):
+        v = cls.__new__(cls)
+        v.masks = _bits(mask)
+        return v
+
+    def __len__(self):
+        return len(self.masks)
+
+    def __getitem__(self, key):
+        if isinstance(key, int):
+            return self.masks[key]
+        if isinstance(key, slice):
+            if key != slice(None):
+                raise TypeError("only the full range [:] is supported")
+            sel = range(len(self.masks))
+        elif callable(key):
+            sel = [i for i in range(len(self.masks)) if key(i)]
+        else:
+            sel = [i for i in range(len(self.masks)) if i in set(key)]
+        out = 0
+        for i in sel:
+            out |= self.masks[i]
+        return out
+
+    def __repr__(self):
+        return "[" + ", ".join("%032x" % m for m in self.masks) + "]"
 
 
 class QReg:
@@ -173,6 +255,84 @@ class QReg:
 
     def sync(self):
         check(lib.qvnt_reg_sync(self._h))
+
+    # -------------------------------------------- reset, algebra, readback
+    @classmethod
+    def _adopt(cls, h, q_num, prec, seed=DEFAULT_SEED):
+        r = cls.__new__(cls)
+        r._h, r.q_num, r.prec, r.seed, r.stream, r._draws = h, q_num, prec, seed, 0, 0
+        return r
+
+    def reset(self, i_state=0):
+        """QReg::reset (quant.rs:202-205)."""
+        check(lib.qvnt_reg_reset(self._h, i_state))
+
+    def reset_by_mask(self, mask):
+        """QReg::reset_by_mask (quant.rs:207-229): zero idx & mask != 0, then normalize."""
+        check(lib.qvnt_reg_reset_by_mask(self._h, mask))
+
+    def __mul__(self, other):
+        """Tensor product (quant.rs:330-371): self on the low qubits; both are kept."""
+        h = ctypes.c_void_p()
+        check(lib.qvnt_reg_tensor(self._h, other.handle, ctypes.byref(h)))
+        return QReg._adopt(h, self.q_num + other.q_num, self.prec, self.seed)
+
+    def __imul__(self, other):
+        out = self * other
+        self.close()
+        self._h, self.q_num = out._h, out.q_num
+        out._h = None
+        return self
+
+    def combine(self, other):
+        """QReg::combine (quant.rs:245-268): self, then other, one qubit larger."""
+        h = ctypes.c_void_p()
+        check(lib.qvnt_reg_combine(self._h, other.handle, ctypes.byref(h)))
+        return QReg._adopt(h, self.q_num + 1, self.prec, self.seed)
+
+    def combine_with_unitary(self, other, m):
+        """QReg::combine_with_unitary (quant.rs:271-308); m: 2x2 complex."""
+        c = np.ascontiguousarray(np.asarray(m, np.complex128).reshape(4)).view(np.float64)
+        h = ctypes.c_void_p()
+        check(lib.qvnt_reg_combine_with_unitary(self._h, other.handle,
+                                                c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h)))
+        return QReg._adopt(h, self.q_num + 1, self.prec, self.seed)
+
+    def linear_composition(self, other, c0, c1):
+        """QReg::linear_composition (quant.rs:311-328): self = self * c0 + other * c1."""
+        c = np.array([complex(c0).real, complex(c0).imag, complex(c1).real, complex(c1).imag], np.float64)
+        check(lib.qvnt_reg_linear_composition(self._h, other.handle, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def set_num(self, q_num):
+        """QReg::set_num (quant.rs:156-166): zero-extend or truncate to 2^q_num amplitudes."""
+        check(lib.qvnt_reg_set_num(self._h, q_num))
+        self.q_num = q_num
+
+    def get_polar(self):
+        """QReg::get_polar (quant.rs:417-431): rows (|psi_i|, arg psi_i)."""
+        out = np.empty(2 << self.q_num, np.float64)
+        check(lib.qvnt_reg_get_polar(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(out)))
+        return out.reshape(-1, 2)
+
+    def sample_all(self, count, seed=None, stream=None):
+        """QReg::sample_all (quant.rs:513-594) as an exact multinomial of
+        `count` seeded draws; one count per buffer element."""
+        seed = self.seed if seed is None else seed
+        stream = self.stream + 1 if stream is None else stream
+        out = np.zeros(len(self), np.uint64)
+        check(lib.qvnt_reg_sample_all(self._h, count, seed, stream, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                      len(out)))
+        return out
+
+    def get_vreg(self):
+        """QReg::get_vreg (quant.rs:232-235)."""
+        return VReg.from_mask((1 << self.q_num) - 1)
+
+    def get_vreg_by(self, mask):
+        """QReg::get_vreg_by (quant.rs:237-243): None for bits beyond the register."""
+        if mask & ~((1 << self.q_num) - 1):
+            return None
+        return VReg.from_mask(mask)
 
     def set_timing(self, enable=True):
         check(lib.qvnt_reg_set_timing(self._h, 1 if enable else 0))
