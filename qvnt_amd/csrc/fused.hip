@@ -81,12 +81,57 @@ __device__ __forceinline__ void apply_m4(cplx<R>* v, const double* __restrict__ 
     }
 }
 
+// anti-diagonal 2x2 (u00 = u11 = 0): out0 = u01 x1, out1 = u10 x0
+template <int J, bool MASKED, class R>
+__device__ __forceinline__ void apply_a2(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
+    const cplx<R> u01 = ld_coef<R>(c, 0), u10 = ld_coef<R>(c, 1);
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & (1 << J)) continue;
+        if (MASKED && !((pc >> i) & 1u)) continue;
+        const cplx<R> x0 = v[i], x1 = v[i | (1 << J)];
+        v[i] = mul_c(u01, x1);
+        v[i | (1 << J)] = mul_c(u10, x0);
+    }
+}
+
+// the swap of the two halves (x, controlled x): no arithmetic
+template <int J, bool MASKED, class R>
+__device__ __forceinline__ void apply_x2(cplx<R>* v, uint32_t pc) {
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & (1 << J)) continue;
+        if (MASKED && !((pc >> i) & 1u)) continue;
+        const cplx<R> x0 = v[i];
+        v[i] = v[i | (1 << J)];
+        v[i | (1 << J)] = x0;
+    }
+}
+
+// 4x4 pairing r with r ^ 3 only: out[r] = u[r][r] x[r] + u[r][r^3] x[r^3]
+template <int J0, int J1, class R>
+__device__ __forceinline__ void apply_p4(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & ((1 << J0) | (1 << J1))) continue;
+        if (!((pc >> i) & 1u)) continue;
+        const int idx[4] = {i, i | (1 << J0), i | (1 << J1), i | (1 << J0) | (1 << J1)};
+        cplx<R> x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            v[idx[r]] = fma_c(mul_c(ld_coef<R>(c, 2 * r), x[r]), ld_coef<R>(c, 2 * r + 1), x[r ^ 3]);
+    }
+}
+
 // diagonal: one phase per register (host tables, controls folded in)
 template <class R>
 __device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c) {
 #pragma unroll
     for (int i = 0; i < kRegs; ++i) v[i] = mul_c(ld_coef<R>(c, i), v[i]);
 }
+
 
 template <class R>
 __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FOp* __restrict__ ops,
@@ -97,16 +142,21 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
         if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
         const double* c = coef + op.coef;
         if (op.type == OP_D) {
-            // table variant from the non-register operands; a waterfall over
-            // the variants present in the wave keeps the table address uniform
+            // table variant from the non-register operands (uniform branches)
+            const bool ea = (op.la | op.ga) != 0, eb = (op.lb | op.gb) != 0;
             const uint32_t k = ((((tb & op.la) | (tile & op.ga)) != 0) ? 1u : 0u) |
                                ((((tb & op.lb) | (tile & op.gb)) != 0) ? 2u : 0u);
-            for (;;) {
-                const uint32_t kk = __builtin_amdgcn_readfirstlane(k);
-                if (k == kk) {
-                    apply_d(v, c + 2 * kRegs * kk);
-                    break;
-                }
+            // dmode 1: the variant-0 table, then (waves whose operand bit is
+            // set) the ratio table, both uniform loads; else each lane loads
+            // its own variant
+            if (!ea && !eb) {
+                apply_d(v, c);
+            } else if (op.dmode == 1) {
+                apply_d(v, c);
+                const uint32_t var = ea ? 1u : 2u;
+                if (k & var) apply_d(v, c + 2 * kRegs * var);  // uniform branch
+            } else {
+                apply_d(v, c + 2 * kRegs * k);  // per-lane variant table
             }
         } else if (op.type == OP_M2) {
             if (op.pc == 0xFFFFu) {
@@ -124,7 +174,7 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
                 default: apply_m2<3, true>(v, c, op.pc); break;
                 }
             }
-        } else {
+        } else if (op.type == OP_M4) {
             switch (op.sel) {
             case 1: apply_m4<0, 1>(v, c, op.pc); break;
             case 2: apply_m4<0, 2>(v, c, op.pc); break;
@@ -132,6 +182,47 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
             case 6: apply_m4<1, 2>(v, c, op.pc); break;
             case 7: apply_m4<1, 3>(v, c, op.pc); break;
             default: apply_m4<2, 3>(v, c, op.pc); break;
+            }
+        } else if (op.type == OP_P4) {
+            switch (op.sel) {
+            case 1: apply_p4<0, 1>(v, c, op.pc); break;
+            case 2: apply_p4<0, 2>(v, c, op.pc); break;
+            case 3: apply_p4<0, 3>(v, c, op.pc); break;
+            case 6: apply_p4<1, 2>(v, c, op.pc); break;
+            case 7: apply_p4<1, 3>(v, c, op.pc); break;
+            default: apply_p4<2, 3>(v, c, op.pc); break;
+            }
+        } else if (op.type == OP_A2) {
+            if (op.pc == 0xFFFFu) {
+                switch (op.sel) {
+                case 0: apply_a2<0, false>(v, c, 0u); break;
+                case 1: apply_a2<1, false>(v, c, 0u); break;
+                case 2: apply_a2<2, false>(v, c, 0u); break;
+                default: apply_a2<3, false>(v, c, 0u); break;
+                }
+            } else {
+                switch (op.sel) {
+                case 0: apply_a2<0, true>(v, c, op.pc); break;
+                case 1: apply_a2<1, true>(v, c, op.pc); break;
+                case 2: apply_a2<2, true>(v, c, op.pc); break;
+                default: apply_a2<3, true>(v, c, op.pc); break;
+                }
+            }
+        } else {
+            if (op.pc == 0xFFFFu) {
+                switch (op.sel) {
+                case 0: apply_x2<0, false>(v, 0u); break;
+                case 1: apply_x2<1, false>(v, 0u); break;
+                case 2: apply_x2<2, false>(v, 0u); break;
+                default: apply_x2<3, false>(v, 0u); break;
+                }
+            } else {
+                switch (op.sel) {
+                case 0: apply_x2<0, true>(v, op.pc); break;
+                case 1: apply_x2<1, true>(v, op.pc); break;
+                case 2: apply_x2<2, true>(v, op.pc); break;
+                default: apply_x2<3, true>(v, op.pc); break;
+                }
             }
         }
     }

@@ -289,16 +289,20 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
             for (int c = 0; c < kRegs; ++c) G[r * kRegs + c] = r == c ? 1.0 : 0.0;
         if (op.type == OP_D) {
             for (int i = 0; i < kRegs; ++i) G[i * kRegs + i] = cf(i);
-        } else if (op.type == OP_M2) {
+        } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2) {
             const int J = (int)op.sel;
+            const cd one(1.0, 0.0), zero(0.0, 0.0);
+            const cd u00 = op.type == OP_M2 ? cf(0) : zero, u11 = op.type == OP_M2 ? cf(3) : zero;
+            const cd u01 = op.type == OP_M2 ? cf(1) : op.type == OP_A2 ? cf(0) : one;
+            const cd u10 = op.type == OP_M2 ? cf(2) : op.type == OP_A2 ? cf(1) : one;
             for (int i = 0; i < kRegs; ++i) {
                 if (i & (1 << J)) continue;
                 if (op.pc != 0xFFFFu && !((op.pc >> i) & 1u)) continue;
                 const int i1 = i | (1 << J);
-                G[i * kRegs + i] = cf(0);
-                G[i * kRegs + i1] = cf(1);
-                G[i1 * kRegs + i] = cf(2);
-                G[i1 * kRegs + i1] = cf(3);
+                G[i * kRegs + i] = u00;
+                G[i * kRegs + i1] = u01;
+                G[i1 * kRegs + i] = u10;
+                G[i1 * kRegs + i1] = u11;
             }
         } else {
             const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;
@@ -307,7 +311,13 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
                 if (!((op.pc >> i) & 1u)) continue;
                 const int idx[4] = {i, i | (1 << j0), i | (1 << j1), i | (1 << j0) | (1 << j1)};
                 for (int r = 0; r < 4; ++r)
-                    for (int c = 0; c < 4; ++c) G[idx[r] * kRegs + idx[c]] = cf(4 * r + c);
+                    for (int c = 0; c < 4; ++c) {
+                        cd v = 0.0;
+                        if (op.type == OP_M4) v = cf(4 * r + c);
+                        else if (c == r) v = cf(2 * r);
+                        else if (c == (r ^ 3)) v = cf(2 * r + 1);
+                        G[idx[r] * kRegs + idx[c]] = v;
+                    }
             }
         }
         cd T[kRegs * kRegs];
@@ -319,6 +329,115 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
             }
         for (int k = 0; k < kRegs * kRegs; ++k) U[k] = T[k];
     }
+}
+
+uint32_t reg_targets(const FOp& op) {
+    if (op.type == OP_D) return 0;
+    if (op.type == OP_M4 || op.type == OP_P4) return (1u << (op.sel / 4)) | (1u << (op.sel % 4));
+    return 1u << op.sel;
+}
+
+// Register bits a diagonal's tables depend on.
+uint32_t diag_deps(const Program& prog, const FOp& op) {
+    uint32_t dep = 0;
+    const double* t = prog.coef.data() + op.coef;
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < kRegs; ++i)
+            for (int j = 0; j < kRegBits; ++j) {
+                const int i2 = i ^ (1 << j);
+                const double* a = t + 2 * (k * kRegs + i);
+                const double* b = t + 2 * (k * kRegs + i2);
+                if (a[0] != b[0] || a[1] != b[1]) dep |= 1u << j;
+            }
+    return dep;
+}
+
+bool has_ext(const FOp& op) { return op.la || op.lb || op.ga || op.gb; }
+
+// Merges diagonal `d` (later) into diagonal `e` (earlier) if their controls
+// and non-register operands allow one table set; true on success.
+bool merge_diag(Program& prog, FOp& e, const FOp& d) {
+    if (e.lctrl != d.lctrl || e.gctrl != d.gctrl) return false;
+    const bool xe = has_ext(e), xd = has_ext(d);
+    if (xe && xd && (e.la != d.la || e.lb != d.lb || e.ga != d.ga || e.gb != d.gb)) return false;
+    double* te = prog.coef.data() + e.coef;
+    const double* td = prog.coef.data() + d.coef;
+    // all products first: without operands of its own, e's variant 0 is
+    // read for every k
+    cd out[4 * kRegs];
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < kRegs; ++i) {
+            const int ke = xe ? k : 0, kd = xd ? k : 0;
+            const cd a(te[2 * (ke * kRegs + i)], te[2 * (ke * kRegs + i) + 1]);
+            const cd b(td[2 * (kd * kRegs + i)], td[2 * (kd * kRegs + i) + 1]);
+            out[k * kRegs + i] = b * a;
+        }
+    for (int k = 0; k < 4 * kRegs; ++k) {
+        te[2 * k] = out[k].real();
+        te[2 * k + 1] = out[k].imag();
+    }
+    if (!xe && xd) {
+        e.la = d.la;
+        e.lb = d.lb;
+        e.ga = d.ga;
+        e.gb = d.gb;
+    }
+    return true;
+}
+
+// A diagonal with exactly one non-register operand: its variant slot
+// (1 for a, 2 for b) becomes T_var / T_0 (the kernel multiplies by T_0,
+// then by the ratio where the operand bit is 1).
+void ratio_table(Program& prog, FOp& op, const FPhase& ph) {
+    if (op.type != OP_D) return;
+    const bool ea = op.la || op.ga, eb = op.lb || op.gb;
+    if (ea == eb) return;
+    // the operand's bit must be uniform over a wave: outside the tile, or a
+    // local position whose thread-index bit (register positions removed) is >= 6
+    const uint32_t lm = ea ? op.la : op.lb;
+    if (lm) {
+        const uint32_t l = ctz(lm);
+        uint32_t below = 0;
+        for (int k = 0; k < kRegBits; ++k) below += ph.rpos[k] < l ? 1u : 0u;
+        if (l - below < 6) return;
+    }
+    op.dmode = 1;
+    const int var = ea ? 1 : 2;
+    double* t = prog.coef.data() + op.coef;
+    for (int i = 0; i < kRegs; ++i) {
+        const cd t0(t[2 * i], t[2 * i + 1]);
+        const cd tv(t[2 * (var * kRegs + i)], t[2 * (var * kRegs + i) + 1]);
+        const cd q = tv / t0;
+        t[2 * (var * kRegs + i)] = q.real();
+        t[2 * (var * kRegs + i) + 1] = q.imag();
+    }
+}
+
+// Moves every diagonal of the phase's ops back past the ops it commutes
+// with (matrix ops on register bits its tables do not depend on; any
+// diagonal) and multiplies it into an earlier diagonal where possible.
+void merge_diagonals(Program& prog, uint32_t begin) {
+    std::vector<FOp> out;
+    for (size_t o = begin; o < prog.ops.size(); ++o) {
+        const FOp op = prog.ops[o];
+        bool merged = false;
+        if (op.type == OP_D) {
+            const uint32_t dep = diag_deps(prog, op);
+            for (size_t j = out.size(); j-- > 0;) {
+                if (out[j].type == OP_D) {
+                    if (merge_diag(prog, out[j], op)) {
+                        merged = true;
+                        break;
+                    }
+                    continue;
+                }
+                if (reg_targets(out[j]) & dep) break;
+            }
+        }
+        if (!merged) out.push_back(op);
+    }
+    prog.ops.resize(begin);
+    prog.ops.insert(prog.ops.end(), out.begin(), out.end());
 }
 
 void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
@@ -409,7 +528,18 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
                 op.coef = add_coef(prog, tab, 4 * kRegs);
             } else if (p.type == OP_M2) {
                 op.sel = (uint32_t)regbit(p.q0);
-                op.coef = add_coef(prog, p.m, 4);
+                if (p.m[0] == 0.0 && p.m[3] == 0.0) {
+                    if (p.m[1] == 1.0 && p.m[2] == 1.0) {
+                        op.type = OP_X2;
+                        op.coef = 0;
+                    } else {
+                        op.type = OP_A2;
+                        const cd a[2] = {p.m[1], p.m[2]};
+                        op.coef = add_coef(prog, a, 2);
+                    }
+                } else {
+                    op.coef = add_coef(prog, p.m, 4);
+                }
             } else {
                 int j0 = regbit(p.q0), j1 = regbit(p.q1);
                 cd u[16];
@@ -422,10 +552,28 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
                     std::swap(j0, j1);
                 }
                 op.sel = (uint32_t)(4 * j0 + j1);
-                op.coef = add_coef(prog, u, 16);
+                bool paired = true;  // u[r][c] == 0 unless c is r or r ^ 3
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c)
+                        if (c != r && c != (r ^ 3) && u[4 * r + c] != 0.0) paired = false;
+                if (paired) {
+                    op.type = OP_P4;
+                    cd pr[8];
+                    for (int r = 0; r < 4; ++r) {
+                        pr[2 * r] = u[4 * r + r];
+                        pr[2 * r + 1] = u[4 * r + (r ^ 3)];
+                    }
+                    op.coef = add_coef(prog, pr, 8);
+                } else {
+                    op.coef = add_coef(prog, u, 16);
+                }
             }
             if (op.lctrl || op.gctrl || op.la || op.lb || op.ga || op.gb) foldable = false;
             prog.ops.push_back(op);
+        }
+        if (!foldable) {
+            merge_diagonals(prog, ph.op_begin);
+            for (size_t o = ph.op_begin; o < prog.ops.size(); ++o) ratio_table(prog, prog.ops[o], ph);
         }
         ph.op_end = (uint32_t)prog.ops.size();
         if (foldable) {

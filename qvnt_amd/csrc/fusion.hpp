@@ -37,25 +37,35 @@ constexpr int kMinFused = 6;               // below 2^6 amplitudes the plain ker
 constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane waves: 2^(t-4) >= 64
 
 // ---------------------------------------------------------- the program
-enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2 };
+// M2: 2x2 on one register bit; M4: 4x4 on two; D: diagonal table;
+// P4: a 4x4 that only pairs index r with r ^ 3 (rxx, ryy, the swap family):
+// each output is two products instead of four; A2: an anti-diagonal 2x2
+// (y, or x with a phase); X2: the pure swap (x, cx).
+enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5 };
 
 // All program fields are 32/64-bit and uniform across a block, so the
 // kernel reads them with scalar loads into SGPRs.
 struct FOp {                // 64 bytes
     uint32_t type;          // OP_M2 / OP_M4 / OP_D
-    uint32_t sel;           // M2: register bit j0; M4: 4*j0 + j1 (j0 < j1)
-    uint32_t pc;            // M2/M4: register indices that satisfy the register controls (0xFFFF: all)
+    uint32_t sel;           // M2/A2/X2: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
+    uint32_t pc;            // matrix ops: register indices that satisfy the register controls (0xFFFF: all)
     uint32_t lctrl;         // controls on lane bits (local positions)
     uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)
     uint32_t coef;          // offset (in doubles) into the coefficient array
-    uint32_t pad;
+    uint32_t dmode;         // D: 0 per-lane variant table, 1 ratio table (see below)
     uint64_t gctrl;         // controls on bits outside the tile (global positions)
     uint64_t ga, gb;        // D: operand a / b when it lies outside the tile
     uint64_t pad2;
 };
 // D coefficients: 4 variants (k = [a is a non-register 1] | 2 [b is a
 // non-register 1]) of the 16 per-register phases, register controls
-// folded in (phase 1 where they fail).  M2: 4 complex; M4: 16 complex.
+// folded in (phase 1 where they fail).  dmode 1 (exactly one non-register
+// operand, and its bit is the same for all 64 lanes of a wave: a bit
+// outside the tile or a thread-index bit >= 6): the operand's variant slot
+// (1 or 2) holds the ratio T_v / T_0, applied after T_0 by the waves whose
+// operand bit is set -- every table load is uniform.  dmode 0: each lane
+// reads its own variant's table.  M2: 4 complex; M4: 16 complex;
+// P4: (u[r][r], u[r][r^3]) for r = 0..3; A2: (u01, u10); X2: none.
 
 struct FPhase {
     uint32_t op_begin, op_end;
