@@ -322,9 +322,14 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
     for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
     const uint64_t base = tile | lo_off;
 
+    // every amplitude of the tile is read once and written once per pass:
+    // non-temporal accesses (streamed through the caches, evicted first)
     cplx<R> v[kRegs];
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) v[j] = psi[base + pass.hoff[j]];
+    for (int j = 0; j < kRegs; ++j) {
+        const cplx<R>* p = psi + base + pass.hoff[j];
+        v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+    }
 
     // the direct layout: register j <-> local tid + j * nth
     const uint32_t sdir = dswz(tid);
@@ -378,7 +383,11 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
         }
     }
 #pragma unroll
-    for (int j = 0; j < kRegs; ++j) psi[base + pass.hoff[j]] = v[j];
+    for (int j = 0; j < kRegs; ++j) {
+        cplx<R>* p = psi + base + pass.hoff[j];
+        __builtin_nontemporal_store(v[j].re, &p->re);
+        __builtin_nontemporal_store(v[j].im, &p->im);
+    }
 }
 
 template <class R>
