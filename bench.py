@@ -345,7 +345,7 @@ def batch(args):
                       "rounds": info["rounds"], "launches": info["launches"], "passes": info["passes"],
                       "tiles": info["tiles"], "phases": info["phases"], "dense_phases": info["dense_phases"],
                       "tile_cap": info["tile_cap"], "dense_min": info["dense_min"], "tune_ms": info["tune_ms"],
-                      "dense_ms": info["dense_ms"],
+                      "single_ms": info["single_ms"],
                       "max_abs_err_vs_plain": max(errs) if errs else None, "checked_registers": len(errs)}
     line = {"metric": METRIC, "value": round(value, 1), "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True, "scaling": "weak",

@@ -371,11 +371,11 @@ typedef struct {
     uint64_t dense_phases;    /* of which dense 16x16 phases on the matrix cores */
     uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
     uint32_t dense_min[32];   /* dense-phase threshold used for each qubit count (0: none) */
-    double tune_ms[32][5];    /* autotuner: ms per run of the registers of each qubit count
-                                 with caps 8, 9, 10, 11, 12, no dense phases (0: not timed or
-                                 the same as a smaller cap) */
-    double dense_ms[32][4];   /* autotuner: the same at the chosen cap with dense thresholds
-                                 2, 3, 4, 6 (0: not timed) */
+    double tune_ms[3][3];     /* autotuner: ms per run of the whole batch with one tile cap
+                                 (10, 11, 12) and one dense threshold (0, 2, 3) for every size
+                                 (0: not timed) */
+    double single_ms[32];     /* autotuner: ms per run with registers of this size in one tile
+                                 each, the others as chosen (0: not timed) */
 } qvnt_batch_info;
 /* Registers k = 0..count-1 of q_nums[k] qubits, each set to basis state
  * states[k] (null: all 0). */
@@ -386,9 +386,10 @@ int qvnt_batch_reset(qvnt_batch* b);
 /* Compiles ops[k] (queue order, as qvnt_reg_apply) for register k.
  * max_qubits > 0 caps every tile at that many qubits (6..12) and folds
  * phases of >= dense_min ops into dense 16x16 unitaries (0: none; see
- * qvnt_reg_set_dense).  max_qubits = 0 runs the autotuner instead: for the
- * registers of each size it times every tile cap, then dense thresholds at
- * the best cap, and it leaves the registers reset to their initial states. */
+ * qvnt_reg_set_dense).  max_qubits = 0 runs the autotuner instead: it times
+ * the whole batch for every (tile cap, dense threshold) pair, then one tile
+ * per register for each size that fits one, keeps the fastest, and leaves
+ * the registers reset to their initial states. */
 int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, uint32_t dense_min,
                        qvnt_batch_info* info);
 /* Queues one application of every compiled circuit (graph != 0: as one

@@ -14,11 +14,11 @@ class BatchInfo(ctypes.Structure):
                 ("tiles", ctypes.c_uint64), ("single_ops", ctypes.c_uint64), ("bytes", ctypes.c_double),
                 ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64),
                 ("tile_cap", ctypes.c_uint32 * 32), ("dense_min", ctypes.c_uint32 * 32),
-                ("tune_ms", (ctypes.c_double * 5) * 32), ("dense_ms", (ctypes.c_double * 4) * 32)]
+                ("tune_ms", (ctypes.c_double * 3) * 3), ("single_ms", ctypes.c_double * 32)]
 
 
-TUNE_CAPS = (8, 9, 10, 11, 12)
-TUNE_DENSE = (2, 3, 4, 6)
+TUNE_CAPS = (10, 11, 12)
+TUNE_DENSE = (0, 2, 3)
 
 
 def _declare():
@@ -73,19 +73,14 @@ class Batch:
         info = BatchInfo()
         check(lib.qvnt_batch_compile(self._h, arr, max_qubits, dense_min, ctypes.byref(info)))
         sizes = sorted(set(self.q_nums))
-        tune, dtune = {}, {}
-        for n in sizes:
-            row = {c: info.tune_ms[n][i] for i, c in enumerate(TUNE_CAPS) if info.tune_ms[n][i] > 0}
-            if row:
-                tune[n] = row
-            row = {d: info.dense_ms[n][i] for i, d in enumerate(TUNE_DENSE) if info.dense_ms[n][i] > 0}
-            if row:
-                dtune[n] = row
+        tune = {"cap%d_dense%d" % (c, d): round(info.tune_ms[i][j], 3)
+                for i, c in enumerate(TUNE_CAPS) for j, d in enumerate(TUNE_DENSE) if info.tune_ms[i][j] > 0}
+        single = {n: round(info.single_ms[n], 3) for n in sizes if info.single_ms[n] > 0}
         return {"rounds": info.rounds, "launches": info.launches, "passes": info.passes, "tiles": info.tiles,
                 "single_ops": info.single_ops, "bytes": info.bytes, "phases": info.phases,
                 "dense_phases": info.dense_phases,
                 "tile_cap": {n: info.tile_cap[n] for n in sizes}, "dense_min": {n: info.dense_min[n] for n in sizes},
-                "tune_ms": tune, "dense_ms": dtune}
+                "tune_ms": tune, "single_ms": single}
 
     def reset(self):
         check(lib.qvnt_batch_reset(self._h))

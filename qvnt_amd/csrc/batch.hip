@@ -33,8 +33,8 @@ namespace {
 
 constexpr unsigned kMinQubits = qvnt::fusion::kMinFused;
 constexpr unsigned kMaxQubits = 30;
-constexpr unsigned kTuneCaps[] = {8, 9, 10, 11, 12};
-constexpr unsigned kTuneDense[] = {2, 3, 4, 6};
+constexpr unsigned kTuneCaps[] = {10, 11, 12};
+constexpr unsigned kTuneDense[] = {0, 2, 3};
 
 size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -394,7 +394,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
     constexpr size_t kCaps = sizeof(kTuneCaps) / sizeof(kTuneCaps[0]);
     constexpr size_t kDense = sizeof(kTuneDense) / sizeof(kTuneDense[0]);
     uint32_t caps[kMaxQubits + 1], dense[kMaxQubits + 1];
-    double tune_ms[kMaxQubits + 1][kCaps] = {}, dense_ms[kMaxQubits + 1][kDense] = {};
+    double tune_ms[kCaps][kDense] = {}, single_ms[kMaxQubits + 1] = {};
     int bad = -1;
     auto no_batched_form = [&] {
         return fail(QVNT_E_ARG, "register " + std::to_string(bad) + " holds a gate with no batched form");
@@ -406,61 +406,58 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
             dense[n] = dense_min;
         }
     } else {
-        // autotune, per register size: every tile cap without dense phases,
-        // then dense thresholds at the best cap; each candidate is timed on
-        // the batch's own registers of that size
+        // autotune on the whole batch (every candidate compiled and timed as
+        // the batch will run: all sizes together, round by round):
+        //  1. one tile cap and one dense threshold for every size, over the grid;
+        //  2. for each size that fits one tile, its own cap = n (one pass), kept
+        //     if the whole batch gets faster.
+        auto time_all = [&](const uint32_t* tc, const uint32_t* td, const std::vector<uint32_t>& recompile,
+                            double* ms) -> int {
+            compile_plans(gates, b->q, recompile, tc, td, plans, &bad);
+            if (bad >= 0) return no_batched_form();
+            Schedule S;
+            if (int rc = build_schedule(b, plans, all, S)) return rc;
+            const int rc = time_schedule(b, S, 2, ms);
+            S.release();
+            return rc;
+        };
+        double best = 1e300;
+        uint32_t tc[kMaxQubits + 1], td[kMaxQubits + 1];
+        for (size_t ci = 0; ci < kCaps; ++ci)
+            for (size_t di = 0; di < kDense; ++di) {
+                if (kTuneCaps[ci] < (unsigned)qvnt::fusion::kMinDenseTile && kTuneDense[di]) continue;  // no dense tiles
+                for (unsigned n = 0; n <= kMaxQubits; ++n) {
+                    tc[n] = kTuneCaps[ci];
+                    td[n] = kTuneDense[di];
+                }
+                double ms = 0.0;
+                if (int rc = time_all(tc, td, all, &ms)) return rc;
+                tune_ms[ci][di] = ms;
+                if (ms < best) {
+                    best = ms;
+                    std::memcpy(caps, tc, sizeof(tc));
+                    std::memcpy(dense, td, sizeof(td));
+                }
+            }
+        // plans must match caps/dense again before the per-size trials
+        compile_plans(gates, b->q, all, caps, dense, plans, &bad);
+        if (bad >= 0) return no_batched_form();
         std::map<uint32_t, std::vector<uint32_t>> by_q;
         for (uint32_t k = 0; k < b->count; ++k) by_q[b->q[k]].push_back(k);
-        std::vector<double> best(kMaxQubits + 1, 1e300);
-        for (unsigned n = 0; n <= kMaxQubits; ++n) {
-            caps[n] = qvnt::fusion::kMaxTile;
-            dense[n] = 0;
-        }
-        auto trial_run = [&](const uint32_t* tc, const uint32_t* td, auto&& want, auto&& record) -> int {
-            compile_plans(gates, b->q, all, tc, td, plans, &bad);
-            if (bad >= 0) return no_batched_form();
-            for (auto& kv : by_q) {
-                if (!want(kv.first)) continue;
-                Schedule S;
-                if (int rc = build_schedule(b, plans, kv.second, S)) return rc;
-                double ms = 0.0;
-                const int rc = time_schedule(b, S, 2, &ms);
-                S.release();
-                if (rc) return rc;
-                record(kv.first, ms);
+        for (auto& kv : by_q) {
+            const unsigned n = kv.first;
+            if (n > (unsigned)qvnt::fusion::kMaxTile || n <= caps[n]) continue;
+            std::memcpy(tc, caps, sizeof(tc));
+            tc[n] = n;
+            double ms = 0.0;
+            if (int rc = time_all(tc, dense, kv.second, &ms)) return rc;
+            single_ms[n] = ms;
+            if (ms < best) {
+                best = ms;
+                caps[n] = n;
+            } else {
+                compile_plans(gates, b->q, kv.second, caps, dense, plans, &bad);  // back to the kept plans
             }
-            return QVNT_OK;
-        };
-        uint32_t tc[kMaxQubits + 1], td[kMaxQubits + 1];
-        for (size_t ci = 0; ci < kCaps; ++ci) {
-            for (unsigned n = 0; n <= kMaxQubits; ++n) {
-                tc[n] = kTuneCaps[ci];
-                td[n] = 0;
-            }
-            const int rc = trial_run(
-                tc, td,
-                [&](unsigned n) { return !(kTuneCaps[ci] > n && ci > 0 && kTuneCaps[ci - 1] >= n); },  // same effective cap
-                [&](unsigned n, double ms) {
-                    tune_ms[n][ci] = ms;
-                    if (ms < best[n]) {
-                        best[n] = ms;
-                        caps[n] = kTuneCaps[ci];
-                    }
-                });
-            if (rc) return rc;
-        }
-        for (size_t di = 0; di < kDense; ++di) {
-            for (unsigned n = 0; n <= kMaxQubits; ++n) td[n] = kTuneDense[di];
-            const int rc = trial_run(
-                caps, td, [&](unsigned n) { return std::min(caps[n], n) >= (unsigned)qvnt::fusion::kMinDenseTile; },
-                [&](unsigned n, double ms) {
-                    dense_ms[n][di] = ms;
-                    if (ms < best[n]) {
-                        best[n] = ms;
-                        dense[n] = kTuneDense[di];
-                    }
-                });
-            if (rc) return rc;
         }
         if (int rc = reset(b)) return rc;
     }
@@ -482,9 +479,10 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         for (unsigned n = 0; n <= kMaxQubits && n < 32; ++n) {
             info->tile_cap[n] = caps[n];
             info->dense_min[n] = dense[n];
-            for (size_t ci = 0; ci < kCaps; ++ci) info->tune_ms[n][ci] = tune_ms[n][ci];
-            for (size_t di = 0; di < kDense; ++di) info->dense_ms[n][di] = dense_ms[n][di];
+            info->single_ms[n] = single_ms[n];
         }
+        for (size_t ci = 0; ci < kCaps; ++ci)
+            for (size_t di = 0; di < kDense; ++di) info->tune_ms[ci][di] = tune_ms[ci][di];
     }
     return QVNT_OK;
 }

@@ -61,12 +61,10 @@ def test_autotune_and_reset():
     regs, states, ops = workload(48, 3, n_lo=8, n_hi=14)
     with Batch([n for n, _ in regs], states) as b:
         info = b.compile(ops)                 # autotuner: runs the batch, then resets it
+        assert len(info["tune_ms"]) == len(TUNE_CAPS) * len(TUNE_DENSE)
         for n, cap in info["tile_cap"].items():
-            assert cap in TUNE_CAPS
-            assert info["tune_ms"][n], "qubit count %d was not timed" % n
-            assert info["dense_min"][n] in (0,) + TUNE_DENSE
-            if min(cap, n) >= 10:
-                assert info["dense_ms"][n], "dense thresholds were not timed for %d qubits" % n
+            assert cap in TUNE_CAPS or cap == n
+            assert info["dense_min"][n] in TUNE_DENSE
         b.run()
         got = [b.get_state(k) for k in range(len(regs))]
     for k, (n, _) in enumerate(regs):
