@@ -232,7 +232,7 @@ def serve(args):
         res = []
         with Server(arena, slots=slots) as srv:
             for k, (n, _, mask, draws) in enumerate(reqs):
-                srv.submit(k, n, ops[k], mask, draws, fused=fused)
+                srv.submit(k, n, ops[k], mask, draws, fused=fused, dense_min=args.dense if fused else 0)
                 res += srv.drain(1)
         return res
 
@@ -255,7 +255,7 @@ def serve(args):
             wait = due[k] - srv.now()
             if wait > 0:
                 time.sleep(wait)
-            srv.submit(k, n, ops[k], mask, draws, fused=True)
+            srv.submit(k, n, ops[k], mask, draws, fused=True, dense_min=args.dense)
         res = srv.drain(len(reqs))
         _, _, peak = srv.pending()
     out["served"] = stats(res, "served (rate %.1f/s, %d slots)" % (args.rate, args.slots))
@@ -272,7 +272,7 @@ def serve(args):
             "steps": len(reqs), "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic (seeded serving requests)",
             "config": {"workload": "serving-%d-requests-12..%dq-100..2000-gates-1024-shots" % (len(reqs), args.qubits),
-                       "arena_gib": args.arena_gib},
+                       "arena_gib": args.arena_gib, "dense_min": args.dense},
             "speedup_vs_serial_plain": round(out["served"]["circuits_per_s"] / base, 2) if base else None}
     line.update(out)
     print(json.dumps(line), flush=True)
@@ -383,9 +383,9 @@ def main():
     ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
-    ap.add_argument("--dense", type=int, default=0,
+    ap.add_argument("--dense", type=int, default=None,
                     help="fused forms: fold phases of >= this many ops into dense 16x16 unitaries (0: none; "
-                         "batch: used with --tile-cap, else autotuned)")
+                         "default 0, serve 3; batch: used with --tile-cap, else autotuned)")
     ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
@@ -399,6 +399,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.dense is None:
+        args.dense = 3 if args.workload == "serve" else 0
     if args.qubits is None:
         args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
                        "batch": 20}[args.workload]

@@ -239,6 +239,8 @@ typedef struct qvnt_request {
     const qvnt_op* op;        /* copied at submit                              */
     uint32_t fused;           /* 1: fused form (qvnt_reg_apply_fused); 0: plain */
     uint32_t max_qubits;      /* fused tile qubits, 0 = default                */
+    uint32_t dense_min;       /* fused: dense-phase threshold, 0 = none (qvnt_reg_set_dense) */
+    uint32_t reserved;        /* 0                                             */
     uint64_t mask;            /* measured qubits                               */
     uint64_t shots;           /* 1 .. 2^20                                     */
     const double* draws;      /* shots uniform draws in [0,1), copied          */

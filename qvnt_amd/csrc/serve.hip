@@ -365,7 +365,7 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
     if (q->fused && ns >= (unsigned)fusion::kMinFused) {
         unsigned tq = q->max_qubits ? q->max_qubits : (unsigned)fusion::kMaxTile;
         tq = std::min(std::max(tq, (unsigned)fusion::kMinFused), std::min((unsigned)fusion::kMaxTile, ns));
-        fusion::plan(rec->gates.data(), rec->gates.size(), ns, tq, rec->steps);
+        fusion::plan(rec->gates.data(), rec->gates.size(), ns, tq, rec->steps, q->dense_min);
     } else {
         for (const qvnt_gate& g : rec->gates) {
             fusion::Step st;

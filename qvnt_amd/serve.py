@@ -11,7 +11,8 @@ from .ops import MultiOp
 class Request(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("q_num", ctypes.c_uint32), ("prec", ctypes.c_uint32),
                 ("init_state", ctypes.c_uint64), ("op", ctypes.c_void_p), ("fused", ctypes.c_uint32),
-                ("max_qubits", ctypes.c_uint32), ("mask", ctypes.c_uint64), ("shots", ctypes.c_uint64),
+                ("max_qubits", ctypes.c_uint32), ("dense_min", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("mask", ctypes.c_uint64), ("shots", ctypes.c_uint64),
                 ("draws", ctypes.POINTER(ctypes.c_double)), ("outcomes", ctypes.POINTER(ctypes.c_uint64))]
 
 
@@ -46,13 +47,13 @@ class Server:
         self._h = h
         self._live = {}  # id -> (draws, outcomes, op): kept alive until polled
 
-    def submit(self, rid, q_num, op, mask, draws, fused=True, prec="f64", init_state=0, max_qubits=0):
+    def submit(self, rid, q_num, op, mask, draws, fused=True, prec="f64", init_state=0, max_qubits=0, dense_min=0):
         if not isinstance(op, MultiOp):
             raise TypeError("submit expects a MultiOp")
         draws = np.ascontiguousarray(draws, np.float64)
         outs = np.zeros(len(draws), np.uint64)
         req = Request(rid, q_num, {"f64": F64, "f32": F32}[prec], init_state, op.handle, 1 if fused else 0,
-                      max_qubits, mask, len(draws), draws.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      max_qubits, dense_min, 0, mask, len(draws), draws.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                       outs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
         self._live[rid] = (draws, outs, op)
         try:

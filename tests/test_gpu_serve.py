@@ -25,7 +25,8 @@ def make_requests(count, seed, n_lo=6, n_hi=16, g_lo=20, g_hi=200, shots=256):
 def run(server, reqs, one_at_a_time):
     results = {}
     for k, (n, specs, mask, draws, fused) in enumerate(reqs):
-        server.submit(k, n, circuits.build(specs, op), mask, draws, fused=fused)
+        # every other fused request folds phases into dense 16x16 blocks
+        server.submit(k, n, circuits.build(specs, op), mask, draws, fused=fused, dense_min=3 if k % 2 else 0)
         if one_at_a_time:
             for r in server.drain(1):
                 results[r["id"]] = r
@@ -64,6 +65,7 @@ def test_server_matches_register_path():
     for k, (n, specs, mask, draws, fused) in enumerate(reqs):
         with QReg(n) as r:
             if fused:
+                r.set_dense(3 if k % 2 else 0)
                 r.apply_fused(circuits.build(specs, op))
             else:
                 r.apply(circuits.build(specs, op))
