@@ -11,6 +11,7 @@
 // measurement of a zero state) the facade throws qvnt::Error.
 #pragma once
 
+#include <array>
 #include <complex>
 #include <cstdint>
 #include <optional>
@@ -175,10 +176,53 @@ public:
         seed_ = s;
         draws_ = 0;
     }
+    // fused form: phases of >= min_ops primitives as dense 16x16 blocks (0: never)
+    void set_dense(uint32_t min_ops) { check(qvnt_reg_set_dense(h_, min_ops)); }
+
+    // reset and register algebra (src/register/quant.rs)
+    void reset(uint64_t i_state = 0) { check(qvnt_reg_reset(h_, i_state)); }
+    void reset_by_mask(uint64_t mask) { check(qvnt_reg_reset_by_mask(h_, mask)); }
+    // Mul for QReg: the tensor product, *this on the low qubits
+    QReg operator*(QReg& other) {
+        qvnt_reg* out = nullptr;
+        check(qvnt_reg_tensor(h_, other.h_, &out));
+        return QReg(out, q_num_ + other.q_num_);
+    }
+    QReg combine(QReg& other) {
+        qvnt_reg* out = nullptr;
+        check(qvnt_reg_combine(h_, other.h_, &out));
+        return QReg(out, q_num_ + 1);
+    }
+    QReg combine_with_unitary(QReg& other, const std::array<C, 4>& m) {
+        qvnt_reg* out = nullptr;
+        check(qvnt_reg_combine_with_unitary(h_, other.h_, reinterpret_cast<const double*>(m.data()), &out));
+        return QReg(out, q_num_ + 1);
+    }
+    void linear_composition(QReg& other, C c0, C c1) {
+        const double c[4] = {c0.real(), c0.imag(), c1.real(), c1.imag()};
+        check(qvnt_reg_linear_composition(h_, other.h_, c));
+    }
+    void set_num(uint32_t q_num) {
+        check(qvnt_reg_set_num(h_, q_num));
+        q_num_ = q_num;
+    }
+    // (|psi_i|, arg psi_i) for i < 2^n
+    std::vector<std::pair<double, double>> get_polar() {
+        std::vector<std::pair<double, double>> out(size_t(1) << q_num_);
+        check(qvnt_reg_get_polar(h_, reinterpret_cast<double*>(out.data()), 2 * out.size()));
+        return out;
+    }
+    // exact multinomial histogram of `count` seeded shots, one count per buffer element
+    std::vector<uint64_t> sample_all(uint64_t count, uint64_t stream = 1) {
+        std::vector<uint64_t> hist(len());
+        check(qvnt_reg_sample_all(h_, count, seed_, stream, hist.data(), hist.size()));
+        return hist;
+    }
     qvnt_reg* handle() { return h_; }
 
 private:
     QReg(uint32_t q_num, uint64_t state, qvnt_prec prec) : q_num_(q_num) { check(qvnt_reg_with_state(q_num, state, prec, &h_)); }
+    QReg(qvnt_reg* h, uint32_t q_num) : h_(h), q_num_(q_num) {}
     qvnt_reg* h_ = nullptr;
     uint32_t q_num_ = 0;
     uint64_t seed_ = 0x51564E54;

@@ -68,6 +68,35 @@ static void tensor_reg3() {
     const double want[8] = {0.25, 0.25, 0, 0, 0.25, 0.25, 0, 0};
     for (int i = 0; i < 8; ++i) EXPECT(std::fabs(p[i] - want[i]) < 1e-9);
 }
+// src/register/quant.rs:679-711 (tensor, the reg1 * reg2 part)
+static void tensor_product() {
+    QReg reg1 = QReg::with_state(2, 0b01);
+    QReg reg2 = QReg::with_state(1, 0b1);
+    reg1.apply(op::h(0b01));
+    reg2.apply(op::h(0b01));
+    auto p = (reg1 * reg2).get_probabilities();
+    const double want[8] = {0.25, 0.25, 0, 0, 0.25, 0.25, 0, 0};
+    EXPECT(p.size() == 8);
+    for (int i = 0; i < 8; ++i) EXPECT(std::fabs(p[i] - want[i]) < 1e-9);
+}
+// src/register/quant.rs:713-724 (histogram)
+static void histogram() {
+    QReg q = QReg::with_state(8, 123);
+    q.apply(op::h(255));
+    for (uint64_t k = 0; k < 10; ++k) {
+        auto hist = q.sample_all(2048, k);
+        uint64_t sum = 0;
+        for (uint64_t c : hist) sum += c;
+        EXPECT(hist.size() == 256 && sum == 2048);
+    }
+}
+// src/register/virtl.rs (virtual_regs): reg = new(0) * new(3) * new(2)
+static void virtual_regs_sizes() {
+    QReg a = QReg::new_(0), b = QReg::new_(3), c = QReg::new_(2);
+    QReg ab = a * b;
+    QReg reg = ab * c;
+    EXPECT(reg.num() == 5 && reg.get_probabilities()[0] == 1.0);
+}
 // benches/performance.rs:4-11 (perf_test_single) at 18..20 qubits, plain and fused
 static void bench_workload() {
     for (uint32_t q = 18; q <= 20; ++q) {
@@ -92,6 +121,9 @@ int main(int argc, char** argv) {
         quantum_reg();
         bell();
         tensor_reg3();
+        tensor_product();
+        histogram();
+        virtual_regs_sizes();
         bench_workload();
     }
     std::printf("%s: %d failure(s)\n", mode.c_str(), failures);
