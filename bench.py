@@ -141,6 +141,7 @@ def tfim(args):
         for _ in range(per_call - 1):
             full *= step
         reg = QReg(n, prec=prec)
+        reg.set_dense(args.dense)
         reg.set_timing(True)
         reg.sync()
         passes = 0
@@ -162,7 +163,7 @@ def tfim(args):
             "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": prec, "data": "synthetic (seeded TFIM ring)",
             "config": {"workload": "tfim-ring-%dq-%s" % (n, prec), "qubits": n, "state_gib": (amp << n) / 2**30,
-                       "fused_steps": args.fused_steps},
+                       "fused_steps": args.fused_steps, "dense_min": args.dense},
             "plain": out["plain"], "fused": out["fused"]}
     print(json.dumps(line), flush=True)
 

@@ -8,6 +8,7 @@ from ._lib import QvntError, Gate, KIND, KINDS, lib, last_error
 from .ops import MultiOp, from_gates, op
 from .register import CReg, QReg, VReg, uniform
 from .program import Program, Sym
+from . import qasm
 
 __all__ = ["QvntError", "Gate", "KIND", "KINDS", "MultiOp", "from_gates", "op",
-           "CReg", "QReg", "VReg", "Program", "Sym", "uniform", "lib", "last_error"]
+           "CReg", "QReg", "VReg", "Program", "Sym", "qasm", "uniform", "lib", "last_error"]
