@@ -136,7 +136,7 @@ def tfim(args):
                                      "avg_ms": round(d["ms"] / d["launches"], 3)}}
         log("tfim plain: %d SingleOps in %.2f s -> %.2f gates/s, %s %.0f GB/s; |psi|^2-1 = %.1e"
             % (gates, wall, gates / wall, dom, ach, norm - 1.0))
-    for label, per_call in (("per_step", 1), ("whole", args.fused_steps)):
+    for label, per_call in (("per_step", 1), ("whole", args.fused_steps)) if args.fused_steps > 0 else ():
         full = step.clone()
         for _ in range(per_call - 1):
             full *= step

@@ -245,12 +245,13 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
 
 // Non-temporal accesses when every amplitude's 128-byte line is touched by
 // one load and one store only: always for diagonals; for pairs and quads
-// when the partner offsets are at least 8 amplitudes (the partners lie in
-// other lines, read by other instructions).
+// when the partner offsets are at least one line (8 FP64 or 16 FP32
+// amplitudes: the partners lie in other lines, read by other instructions).
 template <class R>
 int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
+    constexpr uint64_t line_amps = 128 / (2 * sizeof(R));
     const uint64_t partner = cls == GClass::Diag ? 0 : cls == GClass::Pair ? gate.a : (gate.a | gate.b);
-    const bool nt = cls == GClass::Diag || (partner & 7u) == 0;
+    const bool nt = cls == GClass::Diag || (partner & (line_amps - 1)) == 0;
     if (unroll_setting() == 1)
         return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d) : launch_kernel<R, 1, false>(r, gate, cls, groups, d);
     return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d) : launch_kernel<R, 2, false>(r, gate, cls, groups, d);
