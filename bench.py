@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}   # MI355X dense matrix-core peaks
 METRIC = "gates/s + % HBM roofline at 30q FP64; served circuits/s at p50/p99 latency"
 
 
@@ -355,6 +356,10 @@ def batch(args):
                        "registers": len(regs), "single_ops": single, "state_gib": round(state_gib, 2)},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch"},
+            "mfma": {"achieved": round(info["dense_flops"] / (ms_graph * 1e-3) / 1e12, 3),
+                     "peak": MFMA_PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                     "frac": round(info["dense_flops"] / (ms_graph * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS[prec], 4),
+                     "flops_per_run": info["dense_flops"], "note": "dense 16x16 phases (real 32x32 GEMMs) only"},
             "speedup_vs_plain": round(value / out["plain"]["value"], 2) if "plain" in out else None}
     line.update(out)
     print(json.dumps(line), flush=True)

@@ -371,6 +371,8 @@ typedef struct {
     double bytes;             /* algorithmic HBM bytes per run: 2 * amp * 2^n per pass */
     uint64_t phases;          /* phases per run (all passes) */
     uint64_t dense_phases;    /* of which dense 16x16 phases on the matrix cores */
+    double dense_flops;       /* real matrix-core flops per run: 128 per amplitude per dense phase
+                                 (a complex 16x16 as a real 32x32 GEMM) */
     uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
     uint32_t dense_min[32];   /* dense-phase threshold used for each qubit count (0: none) */
     double tune_ms[3][3];     /* autotuner: ms per run of the whole batch with one tile cap

@@ -12,7 +12,7 @@ from .ops import MultiOp
 class BatchInfo(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_uint32), ("launches", ctypes.c_uint32), ("passes", ctypes.c_uint64),
                 ("tiles", ctypes.c_uint64), ("single_ops", ctypes.c_uint64), ("bytes", ctypes.c_double),
-                ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64),
+                ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64), ("dense_flops", ctypes.c_double),
                 ("tile_cap", ctypes.c_uint32 * 32), ("dense_min", ctypes.c_uint32 * 32),
                 ("tune_ms", (ctypes.c_double * 3) * 3), ("single_ms", ctypes.c_double * 32)]
 
@@ -78,7 +78,7 @@ class Batch:
         single = {n: round(info.single_ms[n], 3) for n in sizes if info.single_ms[n] > 0}
         return {"rounds": info.rounds, "launches": info.launches, "passes": info.passes, "tiles": info.tiles,
                 "single_ops": info.single_ops, "bytes": info.bytes, "phases": info.phases,
-                "dense_phases": info.dense_phases,
+                "dense_phases": info.dense_phases, "dense_flops": info.dense_flops,
                 "tile_cap": {n: info.tile_cap[n] for n in sizes}, "dense_min": {n: info.dense_min[n] for n in sizes},
                 "tune_ms": tune, "single_ms": single}
 

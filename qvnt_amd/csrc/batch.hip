@@ -97,6 +97,7 @@ struct Schedule {
     std::vector<Launch> launches;
     uint32_t rounds = 0;
     uint64_t passes = 0, tiles = 0, phases = 0, dense_phases = 0;
+    double dense_amps = 0.0;  // amplitudes transformed by dense phases per run
     double bytes = 0.0;    // algorithmic HBM bytes of one run: 2 * amp * 2^n per pass
     void* dprog = nullptr;
     void* dlaunch = nullptr;
@@ -196,6 +197,7 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
     S.release();
     S.rounds = 0;
     S.passes = S.tiles = S.phases = S.dense_phases = 0;
+    S.dense_amps = 0.0;
     S.bytes = 0.0;
     size_t total = 0;
     std::vector<size_t> base(b->count, 0);
@@ -242,7 +244,11 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
                 S.passes += 1;
                 S.tiles += tiles;
                 S.phases += fp.phase_end - fp.phase_begin;
-                for (uint32_t ph = fp.phase_begin; ph < fp.phase_end; ++ph) S.dense_phases += plans[k].prog.phases[ph].dense;
+                for (uint32_t ph = fp.phase_begin; ph < fp.phase_end; ++ph)
+                    if (plans[k].prog.phases[ph].dense) {
+                        S.dense_phases += 1;
+                        S.dense_amps += (double)(1ull << b->q[k]);
+                    }
                 S.bytes += 2.0 * (double)b->amp() * (double)(1ull << b->q[k]);
             }
             L.blocks = (uint32_t)map.size();
@@ -476,6 +482,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         info->single_ops = single;
         info->phases = b->sched.phases;
         info->dense_phases = b->sched.dense_phases;
+        info->dense_flops = 128.0 * b->sched.dense_amps;
         for (unsigned n = 0; n <= kMaxQubits && n < 32; ++n) {
             info->tile_cap[n] = caps[n];
             info->dense_min[n] = dense[n];
