@@ -183,8 +183,8 @@ int qvnt_reg_apply_gates(qvnt_reg* reg, const qvnt_gate* gates, size_t count);
 /* The fused form of QReg::apply (no reference counterpart: the reference
  * runs one gather pass per SingleOp).  The queue is scheduled into passes;
  * each pass reads and writes the state once and applies every gate it
- * absorbed while tiles of 2^t amplitudes (t = max_qubits, at most 12; 0 =
- * 12) sit in LDS and registers.  Gates are reordered only past gates on
+ * absorbed while tiles of 2^t amplitudes (t = max_qubits, at most 12 for
+ * FP64 and 13 for FP32; 0 = 12) sit in LDS and registers.  Gates are reordered only past gates on
  * disjoint qubits.  Same result as qvnt_reg_apply to rounding (FMA and
  * merged 2x2 products: about 1e-15 per gate).  *passes (may be NULL) gets
  * the number of passes over the state. */
@@ -375,8 +375,10 @@ typedef struct {
                                  (a complex 16x16 as a real 32x32 GEMM) */
     uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
     uint32_t dense_min[32];   /* dense-phase threshold used for each qubit count (0: none) */
+    uint32_t tune_caps[3];    /* the tile caps the autotuner tried: 10, 11, 12 (FP64) or 11, 12, 13 (FP32) */
+    uint32_t reserved;
     double tune_ms[3][3];     /* autotuner: ms per run of the whole batch with one tile cap
-                                 (10, 11, 12) and one dense threshold (0, 2, 3) for every size
+                                 (tune_caps) and one dense threshold (0, 2, 3) for every size
                                  (0: not timed) */
     double single_ms[32];     /* autotuner: ms per run with registers of this size in one tile
                                  each, the others as chosen (0: not timed) */
@@ -388,7 +390,7 @@ uint32_t qvnt_batch_count(const qvnt_batch* b);
 /* Every register back to its initial basis state (stream-ordered). */
 int qvnt_batch_reset(qvnt_batch* b);
 /* Compiles ops[k] (queue order, as qvnt_reg_apply) for register k.
- * max_qubits > 0 caps every tile at that many qubits (6..12) and folds
+ * max_qubits > 0 caps every tile at that many qubits (6..12 FP64, 6..13 FP32) and folds
  * phases of >= dense_min ops into dense 16x16 unitaries (0: none; see
  * qvnt_reg_set_dense).  max_qubits = 0 runs the autotuner instead: it times
  * the whole batch for every (tile cap, dense threshold) pair, then one tile

@@ -14,10 +14,11 @@ class BatchInfo(ctypes.Structure):
                 ("tiles", ctypes.c_uint64), ("single_ops", ctypes.c_uint64), ("bytes", ctypes.c_double),
                 ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64), ("dense_flops", ctypes.c_double),
                 ("tile_cap", ctypes.c_uint32 * 32), ("dense_min", ctypes.c_uint32 * 32),
+                ("tune_caps", ctypes.c_uint32 * 3), ("reserved", ctypes.c_uint32),
                 ("tune_ms", (ctypes.c_double * 3) * 3), ("single_ms", ctypes.c_double * 32)]
 
 
-TUNE_CAPS = (10, 11, 12)
+TUNE_CAPS = (10, 11, 12)          # FP64; FP32 tries 11, 12, 13
 TUNE_DENSE = (0, 2, 3)
 
 
@@ -73,8 +74,8 @@ class Batch:
         info = BatchInfo()
         check(lib.qvnt_batch_compile(self._h, arr, max_qubits, dense_min, ctypes.byref(info)))
         sizes = sorted(set(self.q_nums))
-        tune = {"cap%d_dense%d" % (c, d): round(info.tune_ms[i][j], 3)
-                for i, c in enumerate(TUNE_CAPS) for j, d in enumerate(TUNE_DENSE) if info.tune_ms[i][j] > 0}
+        tune = {"cap%d_dense%d" % (info.tune_caps[i], d): round(info.tune_ms[i][j], 3)
+                for i in range(3) for j, d in enumerate(TUNE_DENSE) if info.tune_ms[i][j] > 0}
         single = {n: round(info.single_ms[n], 3) for n in sizes if info.single_ms[n] > 0}
         return {"rounds": info.rounds, "launches": info.launches, "passes": info.passes, "tiles": info.tiles,
                 "single_ops": info.single_ops, "bytes": info.bytes, "phases": info.phases,

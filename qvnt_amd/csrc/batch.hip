@@ -33,7 +33,7 @@ namespace {
 
 constexpr unsigned kMinQubits = qvnt::fusion::kMinFused;
 constexpr unsigned kMaxQubits = 30;
-constexpr unsigned kTuneCaps[] = {10, 11, 12};
+constexpr unsigned kTuneCaps[] = {10, 11, 12};  // FP64; FP32 tiles are one qubit larger (tune_cap)
 constexpr unsigned kTuneDense[] = {0, 2, 3};
 
 size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -342,7 +342,7 @@ int qvnt_batch_new(uint32_t count, const uint32_t* q_nums, const uint64_t* state
         amps += 1ull << b->q[k];
     }
     b->arena_amps = amps;
-    for (unsigned n = 0; n <= kMaxQubits; ++n) b->caps[n] = qvnt::fusion::kMaxTile;
+    for (unsigned n = 0; n <= kMaxQubits; ++n) b->caps[n] = qvnt::fusion::kDefaultTile;
     if (hipMalloc(&b->arena, amps * b->amp()) != hipSuccess || hipMalloc(&b->dmeta, 2 * count * sizeof(uint64_t)) != hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
@@ -406,7 +406,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         return fail(QVNT_E_ARG, "register " + std::to_string(bad) + " holds a gate with no batched form");
     };
     if (max_qubits) {
-        const unsigned c = std::max(kMinQubits, std::min<unsigned>(max_qubits, qvnt::fusion::kMaxTile));
+        const unsigned c = std::max(kMinQubits, std::min<unsigned>(max_qubits, qvnt::fusion::max_tile(b->prec == QVNT_F64)));
         for (unsigned n = 0; n <= kMaxQubits; ++n) {
             caps[n] = c;
             dense[n] = dense_min;
@@ -429,11 +429,13 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         };
         double best = 1e300;
         uint32_t tc[kMaxQubits + 1], td[kMaxQubits + 1];
+        auto tune_cap = [&](size_t ci) { return kTuneCaps[ci] + (b->prec == QVNT_F64 ? 0u : 1u); };
         for (size_t ci = 0; ci < kCaps; ++ci)
             for (size_t di = 0; di < kDense; ++di) {
-                if (kTuneCaps[ci] < (unsigned)qvnt::fusion::kMinDenseTile && kTuneDense[di]) continue;  // no dense tiles
+                const unsigned cap_ci = tune_cap(ci);
+                if (cap_ci < (unsigned)qvnt::fusion::kMinDenseTile && kTuneDense[di]) continue;  // no dense tiles
                 for (unsigned n = 0; n <= kMaxQubits; ++n) {
-                    tc[n] = kTuneCaps[ci];
+                    tc[n] = cap_ci;
                     td[n] = kTuneDense[di];
                 }
                 double ms = 0.0;
@@ -452,7 +454,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         for (uint32_t k = 0; k < b->count; ++k) by_q[b->q[k]].push_back(k);
         for (auto& kv : by_q) {
             const unsigned n = kv.first;
-            if (n > (unsigned)qvnt::fusion::kMaxTile || n <= caps[n]) continue;
+            if (n > qvnt::fusion::max_tile(b->prec == QVNT_F64) || n <= caps[n]) continue;
             std::memcpy(tc, caps, sizeof(tc));
             tc[n] = n;
             double ms = 0.0;
@@ -488,8 +490,10 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
             info->dense_min[n] = dense[n];
             info->single_ms[n] = single_ms[n];
         }
-        for (size_t ci = 0; ci < kCaps; ++ci)
+        for (size_t ci = 0; ci < kCaps; ++ci) {
+            info->tune_caps[ci] = kTuneCaps[ci] + (b->prec == QVNT_F64 ? 0u : 1u);
             for (size_t di = 0; di < kDense; ++di) info->tune_ms[ci][di] = tune_ms[ci][di];
+        }
     }
     return QVNT_OK;
 }

@@ -391,7 +391,7 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
+__global__ __launch_bounds__(512) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
                                                const FOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ unsigned char smem[];
     fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem));
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void k_fused(cplx<R>* __restrict__ psi, FPass 
 // b - items[map[b]].block0 of item map[b]; registers, passes and programs
 // are addressed by offsets from the two base pointers.
 template <class R>
-__global__ __launch_bounds__(256) void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
+__global__ __launch_bounds__(512) void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
                                                const BItem* __restrict__ items, const uint32_t* __restrict__ map) {
     extern __shared__ unsigned char smem[];
     const BItem it = items[map[blockIdx.x]];
@@ -504,8 +504,9 @@ int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, 
         if (passes) *passes = (uint32_t)n;
         return QVNT_OK;
     }
-    unsigned tq = max_qubits ? max_qubits : (unsigned)kMaxTile;
-    if (tq > (unsigned)kMaxTile) tq = kMaxTile;
+    const unsigned cap = max_tile(r->prec == QVNT_F64);
+    unsigned tq = max_qubits ? max_qubits : kDefaultTile;
+    if (tq > cap) tq = cap;
     if (tq < (unsigned)kMinFused) tq = kMinFused;
     if (tq > r->ns) tq = r->ns;
     std::vector<Step> steps;

@@ -32,7 +32,12 @@ namespace fusion {
 
 constexpr int kRegBits = 4;                // register qubits per phase
 constexpr int kRegs = 1 << kRegBits;       // amplitudes per thread
-constexpr int kMaxTile = 12;               // t: at most 2^12 amplitudes per tile
+constexpr int kMaxTile = 13;               // t: at most 2^13 amplitudes per tile (64 KiB of LDS in FP32)
+// the largest tile whose amplitudes fit the kernel's 64 KiB LDS tile
+constexpr unsigned max_tile(bool fp64) { return fp64 ? 12u : 13u; }
+// the default tile cap: 2^12 amplitudes in both precisions (an FP32 tile of
+// 2^13 is one 512-thread block per CU and measured slower on large states)
+constexpr unsigned kDefaultTile = 12;
 constexpr int kMinFused = 6;               // below 2^6 amplitudes the plain kernels run
 constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane waves: 2^(t-4) >= 64
 

@@ -363,8 +363,9 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
         }
     const unsigned ns = q->q_num < 3 ? 3 : q->q_num;
     if (q->fused && ns >= (unsigned)fusion::kMinFused) {
-        unsigned tq = q->max_qubits ? q->max_qubits : (unsigned)fusion::kMaxTile;
-        tq = std::min(std::max(tq, (unsigned)fusion::kMinFused), std::min((unsigned)fusion::kMaxTile, ns));
+        const unsigned cap = fusion::max_tile(q->prec == QVNT_F64);
+        unsigned tq = q->max_qubits ? q->max_qubits : fusion::kDefaultTile;
+        tq = std::min(std::max(tq, (unsigned)fusion::kMinFused), std::min(cap, ns));
         fusion::plan(rec->gates.data(), rec->gates.size(), ns, tq, rec->steps, q->dense_min);
     } else {
         for (const qvnt_gate& g : rec->gates) {

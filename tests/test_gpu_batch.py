@@ -63,7 +63,7 @@ def test_autotune_and_reset():
         info = b.compile(ops)                 # autotuner: runs the batch, then resets it
         assert len(info["tune_ms"]) == len(TUNE_CAPS) * len(TUNE_DENSE)
         for n, cap in info["tile_cap"].items():
-            assert cap in TUNE_CAPS or cap == n
+            assert cap in TUNE_CAPS or cap == n   # FP64 register set
             assert info["dense_min"][n] in TUNE_DENSE
         b.run()
         got = [b.get_state(k) for k in range(len(regs))]
@@ -101,11 +101,11 @@ def test_graph_replays_match_eager():
         assert np.array_equal(twice, r.get_state()[: 1 << n])
 
 
-@pytest.mark.parametrize("dense", [0, 2])
-def test_fp32_batch_error_bound(dense):
-    regs, states, ops = workload(24, 9 + dense, n_lo=8, n_hi=14)
+@pytest.mark.parametrize("dense,cap", [(0, 12), (2, 12), (0, 13), (3, 13)])
+def test_fp32_batch_error_bound(dense, cap):
+    regs, states, ops = workload(24, 9 + dense + cap, n_lo=8, n_hi=16)
     with Batch([n for n, _ in regs], states, prec="f32") as b:
-        b.compile(ops, max_qubits=12, dense_min=dense)
+        b.compile(ops, max_qubits=cap, dense_min=dense)
         b.run()
         for k, (n, _) in enumerate(regs):
             with QReg.with_state(n, states[k]) as r:

@@ -43,7 +43,7 @@ def test_random_circuits(orc, n, tq):
 
 
 @pytest.mark.parametrize("n,tq,dense,prec", [(10, 10, 2, "f64"), (13, 12, 2, "f64"), (16, 11, 3, "f64"),
-                                             (14, 12, 2, "f32")])
+                                             (14, 12, 2, "f32"), (13, 13, 0, "f32"), (17, 13, 3, "f32")])
 def test_dense_phases(orc, n, tq, dense, prec):
     """Phases folded into dense 16x16 unitaries on the matrix cores."""
     say("dense n=%d tq=%d min_ops=%d %s" % (n, tq, dense, prec))
