@@ -1,0 +1,276 @@
+// A CPU model of the fused kernel (qvnt_amd/csrc/fused.hip: fused_tile,
+// run_phase, dense_phase) driven by the real host compiler
+// (qvnt_amd/csrc/fusion.cpp: fusion::plan).  For random circuits of every
+// gate kind, with controls, it checks the compiled program's structure
+// (indices inside their arrays, tile and phase layouts, the LDS swizzle) and
+// that running it on a random state equals the oracle's plain apply.  Built
+// with address/undefined-behaviour sanitizers by tests/test_host_compiler.py.
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../oracle/qvnt_oracle.h"
+#include "../../qvnt_amd/csrc/fusion.hpp"
+
+using namespace qvnt::fusion;
+typedef std::complex<double> C;
+
+static int failures = 0;
+#define CHECK(c)                                                                     \
+    do {                                                                             \
+        if (!(c)) {                                                                  \
+            std::fprintf(stderr, "CHECK failed %s:%d %s\n", __FILE__, __LINE__, #c); \
+            std::exit(1);                                                            \
+        }                                                                            \
+    } while (0)
+
+static uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
+static C cf(const Program& P, uint32_t off, int k) {
+    CHECK(off + 2 * k + 1 < P.coef.size());
+    return C(P.coef[off + 2 * k], P.coef[off + 2 * k + 1]);
+}
+
+// One register phase of one thread: the ops of fused.hip's run_phase.
+static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uint64_t tile) {
+    CHECK(ph.op_begin <= ph.op_end && ph.op_end <= P.ops.size());
+    for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
+        const FOp& op = P.ops[o];
+        if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
+        if (op.type == OP_D) {
+            const uint32_t k = ((((tb & op.la) | (tile & op.ga)) != 0) ? 1u : 0u) |
+                               ((((tb & op.lb) | (tile & op.gb)) != 0) ? 2u : 0u);
+            const bool ea = op.la || op.ga, eb = op.lb || op.gb;
+            if (op.dmode == 1) {
+                CHECK(ea != eb);
+                const uint32_t var = ea ? 1 : 2;
+                for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
+                if (k & var)
+                    for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, var * kRegs + i) * v[i];
+            } else {
+                for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, k * kRegs + i) * v[i];
+            }
+        } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2) {
+            const int J = (int)op.sel;
+            CHECK(J < kRegBits);
+            for (int i = 0; i < kRegs; ++i) {
+                if (i & (1 << J)) continue;
+                if (op.pc != 0xFFFFu && !((op.pc >> i) & 1u)) continue;
+                const C x0 = v[i], x1 = v[i | (1 << J)];
+                if (op.type == OP_M2) {
+                    v[i] = cf(P, op.coef, 0) * x0 + cf(P, op.coef, 1) * x1;
+                    v[i | (1 << J)] = cf(P, op.coef, 2) * x0 + cf(P, op.coef, 3) * x1;
+                } else if (op.type == OP_A2) {
+                    v[i] = cf(P, op.coef, 0) * x1;
+                    v[i | (1 << J)] = cf(P, op.coef, 1) * x0;
+                } else {
+                    v[i] = x1;
+                    v[i | (1 << J)] = x0;
+                }
+            }
+        } else {
+            CHECK(op.type == OP_M4 || op.type == OP_P4);
+            const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;
+            CHECK(j0 < j1 && j1 < kRegBits);
+            for (int i = 0; i < kRegs; ++i) {
+                if (i & ((1 << j0) | (1 << j1))) continue;
+                if (!((op.pc >> i) & 1u)) continue;
+                const int idx[4] = {i, i | (1 << j0), i | (1 << j1), i | (1 << j0) | (1 << j1)};
+                C x[4], y[4];
+                for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
+                for (int r = 0; r < 4; ++r) {
+                    if (op.type == OP_M4) {
+                        y[r] = 0.0;
+                        for (int k = 0; k < 4; ++k) y[r] += cf(P, op.coef, 4 * r + k) * x[k];
+                    } else {
+                        y[r] = cf(P, op.coef, 2 * r) * x[r] + cf(P, op.coef, 2 * r + 1) * x[r ^ 3];
+                    }
+                }
+                for (int r = 0; r < 4; ++r) v[idx[r]] = y[r];
+            }
+        }
+    }
+}
+
+// One pass over the state: every tile in turn (fused.hip: fused_tile).
+static void run_pass(C* psi, uint64_t len, const Program& P, const FPass& pass) {
+    const uint32_t t = pass.t, nth = pass.nthreads;
+    CHECK(t >= 4 && t <= (uint32_t)kMaxTile && nth == (1u << (t - kRegBits)));
+    for (uint32_t k = 1; k < t; ++k) CHECK(pass.tpos[k - 1] < pass.tpos[k]);
+    CHECK((1ull << pass.tpos[t - 1]) < len);
+    CHECK(pass.phase_begin < pass.phase_end && pass.phase_end <= P.phases.size());
+    std::vector<C> lds(1u << t), V(nth * kRegs);
+    for (uint64_t blk = 0; blk < (len >> t); ++blk) {
+        uint64_t tile = blk;
+        for (uint32_t k = 0; k < t; ++k) {
+            const uint64_t lo = tile & ((1ull << pass.tpos[k]) - 1u);
+            tile = ((tile ^ lo) << 1) | lo;
+        }
+        std::vector<uint64_t> lo_off(nth);
+        for (uint32_t tid = 0; tid < nth; ++tid) {
+            uint64_t lo = 0;
+            for (uint32_t k = 0; k + kRegBits < t; ++k) lo |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
+            lo_off[tid] = lo;
+            for (int j = 0; j < kRegs; ++j) {
+                const uint64_t g = tile | lo | pass.hoff[j];
+                CHECK(g < len);
+                V[tid * kRegs + j] = psi[g];
+            }
+        }
+        for (uint32_t p = pass.phase_begin; p < pass.phase_end; ++p) {
+            const FPhase& ph = P.phases[p];
+            uint32_t roff[kRegs];
+            for (int k = 1; k < kRegBits; ++k) CHECK(ph.rpos[k - 1] < ph.rpos[k]);
+            CHECK(ph.rpos[kRegBits - 1] < t);
+            for (int i = 0; i < kRegs; ++i) {
+                uint32_t off = 0;
+                for (int b = 0; b < kRegBits; ++b)
+                    if (i >> b & 1) off |= 1u << ph.rpos[b];
+                roff[i] = off;
+                CHECK(ph.sroff[i] == swz(off));
+            }
+            const bool din = (p == pass.phase_begin) && pass.direct_first;
+            const bool dout = (p + 1 == pass.phase_end) && pass.direct_last;
+            if (!din && p == pass.phase_begin)
+                for (uint32_t tid = 0; tid < nth; ++tid)
+                    for (int j = 0; j < kRegs; ++j) lds[swz(tid) ^ swz(j * nth)] = V[tid * kRegs + j];
+            if (ph.dense) {
+                CHECK(!din && !dout && t >= (uint32_t)kMinDenseTile && ph.op_begin == ph.op_end);
+                CHECK(ph.ucoef + 2 * kRegs * kRegs <= P.coef.size());
+                for (uint32_t tid = 0; tid < nth; ++tid) {
+                    uint32_t tb = tid;
+                    for (int k = 0; k < kRegBits; ++k) {
+                        const uint32_t lo = tb & ((1u << ph.rpos[k]) - 1u);
+                        tb = ((tb ^ lo) << 1) | lo;
+                    }
+                    C x[kRegs];
+                    for (int j = 0; j < kRegs; ++j) x[j] = lds.at(swz(tb) ^ swz(roff[j]));
+                    for (int m = 0; m < kRegs; ++m) {
+                        C acc = 0.0;
+                        for (int j = 0; j < kRegs; ++j) acc += cf(P, ph.ucoef, kRegs * m + j) * x[j];
+                        lds.at(swz(tb) ^ swz(roff[m])) = acc;
+                    }
+                }
+                if (p + 1 == pass.phase_end)
+                    for (uint32_t tid = 0; tid < nth; ++tid)
+                        for (int j = 0; j < kRegs; ++j) V[tid * kRegs + j] = lds[swz(tid) ^ swz(j * nth)];
+                continue;
+            }
+            for (uint32_t tid = 0; tid < nth; ++tid) {
+                uint32_t tb = tid;
+                for (int k = 0; k < kRegBits; ++k) {
+                    const uint32_t lo = tb & ((1u << ph.rpos[k]) - 1u);
+                    tb = ((tb ^ lo) << 1) | lo;
+                }
+                C* v = &V[tid * kRegs];
+                if (!din)
+                    for (int i = 0; i < kRegs; ++i) v[i] = lds.at(swz(tb) ^ ph.sroff[i]);
+                else
+                    for (int i = 0; i < kRegs; ++i) CHECK((tb | roff[i]) == tid + i * nth);
+                run_phase(v, ph, P, tb, tile);
+                if (!dout)
+                    for (int i = 0; i < kRegs; ++i) lds.at(swz(tb) ^ ph.sroff[i]) = v[i];
+                else
+                    for (int i = 0; i < kRegs; ++i) CHECK((tb | roff[i]) == tid + i * nth);
+            }
+            if (!dout && p + 1 == pass.phase_end)
+                for (uint32_t tid = 0; tid < nth; ++tid)
+                    for (int j = 0; j < kRegs; ++j) V[tid * kRegs + j] = lds[swz(tid) ^ swz(j * nth)];
+        }
+        for (uint32_t tid = 0; tid < nth; ++tid)
+            for (int j = 0; j < kRegs; ++j) psi[tile | lo_off[tid] | pass.hoff[j]] = V[tid * kRegs + j];
+    }
+}
+
+// Random gates of every kind through the oracle's builders, with controls.
+static std::vector<qvnt_gate> random_circuit(std::mt19937_64& g, unsigned n, int count) {
+    std::vector<qvnt_gate> out;
+    std::uniform_real_distribution<double> ang(0.0, 6.283185307179586);
+    auto bit = [&](uint64_t avoid) {
+        for (;;) {
+            const uint64_t b = 1ull << (g() % n);
+            if (!(b & avoid)) return b;
+        }
+    };
+    qvnt_gate buf[64];
+    for (int k = 0; k < count; ++k) {
+        const uint64_t a = bit(0), b = bit(a);
+        long m = 0;
+        switch (g() % 18) {
+        case 0: m = orc_op_x(a | (g() % 3 == 0 ? b : 0), buf); break;
+        case 1: m = orc_op_y(a, buf); break;
+        case 2: m = orc_op_z(a | b, buf); break;
+        case 3: m = orc_op_s(a, buf); break;
+        case 4: m = orc_op_t(a, buf); break;
+        case 5: m = orc_op_rx(ang(g), a, buf); break;
+        case 6: m = orc_op_ry(ang(g), a, buf); break;
+        case 7: m = orc_op_rz(ang(g), a, buf); break;
+        case 8: m = orc_op_rxx(ang(g), a | b, buf); break;
+        case 9: m = orc_op_ryy(ang(g), a | b, buf); break;
+        case 10: m = orc_op_rzz(ang(g), a | b, buf); break;
+        case 11: m = orc_op_h(a | b, buf); break;
+        case 12: m = orc_op_swap(a | b, buf); break;
+        case 13: m = orc_op_i_swap(a | b, buf); break;
+        case 14: m = orc_op_sqrt_swap(a | b, buf); break;
+        case 15: m = orc_op_sqrt_i_swap(a | b, buf); break;
+        case 16: m = orc_op_phi(ang(g), a, buf); break;
+        default: m = orc_op_u3(ang(g), ang(g), ang(g), a, buf); break;
+        }
+        CHECK(m > 0 && m < 64);
+        if (n >= 3 && g() % 3 == 0) {
+            const uint64_t c = bit(orc_op_act_on(buf, (size_t)m));
+            CHECK(orc_op_c(buf, (size_t)m, c) == 0);
+        }
+        if (g() % 11 == 0) orc_op_dgr(buf, (size_t)m);
+        out.insert(out.end(), buf, buf + m);
+    }
+    return out;
+}
+
+static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed) {
+    std::mt19937_64 g(seed);
+    const std::vector<qvnt_gate> gates = random_circuit(g, n, 160);
+    std::vector<Step> steps;
+    plan(gates.data(), gates.size(), n, tq, steps, dense);
+    orc_reg* ref = orc_reg_with_state(n, 0);
+    const uint64_t len = orc_reg_len(ref);
+    std::normal_distribution<double> nd;
+    std::vector<C> psi(len);
+    for (auto& z : psi) z = C(nd(g), nd(g));
+    std::memcpy(orc_reg_psi(ref), psi.data(), len * sizeof(C));
+    orc_reg_apply(ref, gates.data(), gates.size());
+    unsigned passes = 0, dense_phases = 0;
+    for (const Step& st : steps) {
+        if (st.plain) {
+            orc_reg* one = orc_reg_with_state(n, 0);  // a gate the fused kernel does not take
+            std::memcpy(orc_reg_psi(one), psi.data(), len * sizeof(C));
+            orc_reg_apply(one, &st.gate, 1);
+            std::memcpy(psi.data(), orc_reg_psi(one), len * sizeof(C));
+            orc_reg_free(one);
+            continue;
+        }
+        for (const FPhase& ph : st.prog.phases) dense_phases += ph.dense;
+        for (const FPass& fp : st.prog.passes) {
+            run_pass(psi.data(), len, st.prog, fp);
+            ++passes;
+        }
+    }
+    const C* want = reinterpret_cast<const C*>(orc_reg_psi(ref));
+    double err = 0.0;
+    for (uint64_t i = 0; i < len; ++i) err = std::max(err, std::abs(psi[i] - want[i]));
+    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u dense phases, max |d| %.2e\n", n, tq, dense,
+                gates.size(), passes, dense_phases, err);
+    if (!(err <= 1e-12)) ++failures;
+    orc_reg_free(ref);
+}
+
+int main() {
+    const unsigned cases[][3] = {{6, 6, 0}, {7, 6, 0}, {9, 8, 0}, {10, 10, 2}, {11, 9, 0}, {12, 12, 3},
+                                 {13, 12, 2}, {13, 13, 0}, {14, 11, 0}, {14, 13, 3}, {15, 12, 2}};
+    uint64_t seed = 1;
+    for (const auto& c : cases) check_case(c[0], c[1], c[2], seed++);
+    std::printf("%d failure(s)\n", failures);
+    return failures ? 1 : 0;
+}
