@@ -28,9 +28,21 @@ COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+# fused.hip: the op dispatch switches over a register bit, and LLVM's
+# switch-tail sinking merges the cases into one block that indexes the
+# 16-amplitude register array dynamically, which sends the array to
+# scratch memory; without the sinking every case keeps constant indices.
+EXTRA = {"fused.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
+def compile_command(src, obj):
+    """The hipcc command line that builds one source file."""
+    return [HIPCC] + COMMON + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
+
+
 def _compile(src):
     obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = compile_command(src, obj)
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-x", "c++"] + [f for f in COMMON if not f.startswith("--offload-arch")
                                       and f not in ("-Xclang", "-target-feature", "-packed-fp32-ops")] + \

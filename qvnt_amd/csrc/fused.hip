@@ -108,6 +108,40 @@ __device__ __forceinline__ void apply_x2(cplx<R>* v, uint32_t pc) {
     }
 }
 
+// the real middle factor of a split 2x2 (fusion.hpp: OP_T2), one FMA per
+// real output: out0 = x0 - a x1, out1 = x1 + a x0
+template <int J, bool MASKED, class R>
+__device__ __forceinline__ void apply_t2(cplx<R>* v, R a, uint32_t pc) {
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & (1 << J)) continue;
+        if (MASKED && !((pc >> i) & 1u)) continue;
+        const cplx<R> x0 = v[i], x1 = v[i | (1 << J)];
+        v[i] = cplx<R>{fmar(-a, x1.re, x0.re), fmar(-a, x1.im, x0.im)};
+        v[i | (1 << J)] = cplx<R>{fmar(a, x0.re, x1.re), fmar(a, x0.im, x1.im)};
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void apply_t2_sel(cplx<R>* v, const double* __restrict__ c, uint32_t sel, uint32_t pc) {
+    const R a = (R)c[0];
+    if (pc == 0xFFFFu) {
+        switch (sel) {
+        case 0: apply_t2<0, false>(v, a, 0u); break;
+        case 1: apply_t2<1, false>(v, a, 0u); break;
+        case 2: apply_t2<2, false>(v, a, 0u); break;
+        default: apply_t2<3, false>(v, a, 0u); break;
+        }
+    } else {
+        switch (sel) {
+        case 0: apply_t2<0, true>(v, a, pc); break;
+        case 1: apply_t2<1, true>(v, a, pc); break;
+        case 2: apply_t2<2, true>(v, a, pc); break;
+        default: apply_t2<3, true>(v, a, pc); break;
+        }
+    }
+}
+
 // 4x4 pairing r with r ^ 3 only: out[r] = u[r][r] x[r] + u[r][r^3] x[r^3]
 template <int J0, int J1, class R>
 __device__ __forceinline__ void apply_p4(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
@@ -132,16 +166,36 @@ __device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c
     for (int i = 0; i < kRegs; ++i) v[i] = mul_c(ld_coef<R>(c, i), v[i]);
 }
 
+// the structured block (fusion.hpp: OP_S): diagonal, T2 on register bits
+// 0..3, diagonal -- straight-line code, so the 16 amplitudes stay in
+// registers from the first product to the last
+template <class R>
+__device__ __forceinline__ void apply_s(cplx<R>* v, const double* __restrict__ c) {
+    apply_d(v, c);
+    apply_t2<0, false>(v, (R)c[2 * kRegs + 0], 0u);
+    apply_t2<1, false>(v, (R)c[2 * kRegs + 2], 0u);
+    apply_t2<2, false>(v, (R)c[2 * kRegs + 4], 0u);
+    apply_t2<3, false>(v, (R)c[2 * kRegs + 6], 0u);
+    apply_d(v, c + 2 * (kRegs + kRegBits));
+}
+
 
 template <class R>
 __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FOp* __restrict__ ops,
                                           const double* __restrict__ coef, uint32_t tb, uint64_t tile) {
+    // the product of the phase's G factors (uniform, accumulated in FP64)
+    cplx<double> scal{1.0, 0.0};
+    bool scaled = false;
     for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
         const FOp& op = ops[o];
         // controls on lane bits (this thread's base) and on bits outside the tile
         if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
         const double* c = coef + op.coef;
-        if (op.type == OP_D) {
+        if (op.type == OP_G) {
+            const uint32_t k = ((tile & op.ga) != 0 ? 1u : 0u) | ((tile & op.gb) != 0 ? 2u : 0u);
+            scal = mul_c(ld_coef<double>(c, (int)(k * kRegs)), scal);
+            scaled = true;
+        } else if (op.type == OP_D) {
             // table variant from the non-register operands (uniform branches)
             const bool ea = (op.la | op.ga) != 0, eb = (op.lb | op.gb) != 0;
             const uint32_t k = ((((tb & op.la) | (tile & op.ga)) != 0) ? 1u : 0u) |
@@ -158,6 +212,10 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
             } else {
                 apply_d(v, c + 2 * kRegs * k);  // per-lane variant table
             }
+        } else if (op.type == OP_S) {
+            apply_s(v, c);
+        } else if (op.type == OP_T2) {
+            apply_t2_sel(v, c, op.sel, op.pc);
         } else if (op.type == OP_M2) {
             if (op.pc == 0xFFFFu) {
                 switch (op.sel) {
@@ -225,6 +283,11 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
                 }
             }
         }
+    }
+    if (scaled) {
+        const cplx<R> f{(R)scal.re, (R)scal.im};
+#pragma unroll
+        for (int i = 0; i < kRegs; ++i) v[i] = mul_c(f, v[i]);
     }
 }
 
@@ -302,36 +365,26 @@ __device__ __forceinline__ void dense_phase(cplx<R>* lds, const FPhase& ph, cons
     }
 }
 
-// One tile of one pass: read 2^t amplitudes, run the pass's phases, write
-// them back.  `tile_index` enumerates the tiles (the bits outside the
-// tile's qubit set); the pass and program live in scalar-loadable memory.
-template <class R>
-__device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPass& pass, const FPhase* __restrict__ phases,
-                                           const FOp* __restrict__ ops, const double* __restrict__ coef,
-                                           uint64_t tile_index, cplx<R>* lds) {
-    const uint32_t tid = threadIdx.x, nth = pass.nthreads;
-
-    // global index of local 0: insert zeros at the tile's bit positions
+// Global index of local element 0 of tile `tile_index` (the tiles
+// enumerate the bits outside the tile's qubit set): zeros inserted at the
+// tile's bit positions.
+__device__ __forceinline__ uint64_t tile_base(const FPass& pass, uint64_t tile_index) {
     uint64_t tile = tile_index;
     for (uint32_t k = 0; k < pass.t; ++k) {
         const uint64_t lo = tile & ((1ull << pass.tpos[k]) - 1u);
         tile = ((tile ^ lo) << 1) | lo;
     }
-    // global offset of this thread's local bits 0..t-5
-    uint64_t lo_off = 0;
-    for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
-    const uint64_t base = tile | lo_off;
+    return tile;
+}
 
-    // every amplitude of the tile is read once and written once per pass:
-    // non-temporal accesses (streamed through the caches, evicted first)
-    cplx<R> v[kRegs];
-#pragma unroll
-    for (int j = 0; j < kRegs; ++j) {
-        const cplx<R>* p = psi + base + pass.hoff[j];
-        v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
-    }
-
-    // the direct layout: register j <-> local tid + j * nth
+// The phases of one pass on one tile.  On entry and on exit v holds the
+// thread's amplitudes in the direct layout (register j = local tid + j *
+// nth); `lds` is the tile's swizzled LDS copy (local element e at dswz(e)).
+template <class R>
+__device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPass& pass, const FPhase* __restrict__ phases,
+                                            const FOp* __restrict__ ops, const double* __restrict__ coef, uint64_t tile,
+                                            uint32_t tid) {
+    const uint32_t nth = pass.nthreads;
     const uint32_t sdir = dswz(tid);
     const uint32_t p0 = pass.phase_begin, p1 = pass.phase_end;
     for (uint32_t p = p0; p < p1; ++p) {
@@ -382,6 +435,19 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
             }
         }
     }
+}
+
+// global offset of this thread's local bits 0..t-5
+__device__ __forceinline__ uint64_t thread_offset(const FPass& pass, uint32_t tid) {
+    uint64_t lo_off = 0;
+    for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
+    return lo_off;
+}
+
+// every amplitude of the tile is read once and written once per pass:
+// non-temporal stores (streamed through the caches, evicted first)
+template <class R>
+__device__ __forceinline__ void store_direct(cplx<R>* __restrict__ psi, const FPass& pass, uint64_t base, const cplx<R>* v) {
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
         cplx<R>* p = psi + base + pass.hoff[j];
@@ -390,8 +456,32 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
     }
 }
 
+// One tile of one pass: read 2^t amplitudes, run the pass's phases, write
+// them back; the pass and program live in scalar-loadable memory.
 template <class R>
-__global__ __launch_bounds__(512) void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
+__device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPass& pass, const FPhase* __restrict__ phases,
+                                           const FOp* __restrict__ ops, const double* __restrict__ coef,
+                                           uint64_t tile_index, cplx<R>* lds) {
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tile = tile_base(pass, tile_index);
+    const uint64_t base = tile | thread_offset(pass, tid);
+    cplx<R> v[kRegs];
+#pragma unroll
+    for (int j = 0; j < kRegs; ++j) {
+        const cplx<R>* p = psi + base + pass.hoff[j];
+        v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+    }
+    tile_phases<R>(v, lds, pass, phases, ops, coef, tile, tid);
+    store_direct(psi, pass, base, v);
+}
+
+// At most two waves per SIMD: the 64 KiB LDS tile allows no more, and the
+// whole VGPR file then holds the 16 amplitudes and the op operands
+// without spills.
+#define QVNT_FUSED_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2)))
+
+template <class R>
+__global__ QVNT_FUSED_BOUNDS void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
                                                const FOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ unsigned char smem[];
     fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem));
@@ -402,7 +492,7 @@ __global__ __launch_bounds__(512) void k_fused(cplx<R>* __restrict__ psi, FPass 
 // b - items[map[b]].block0 of item map[b]; registers, passes and programs
 // are addressed by offsets from the two base pointers.
 template <class R>
-__global__ __launch_bounds__(512) void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
+__global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
                                                const BItem* __restrict__ items, const uint32_t* __restrict__ map) {
     extern __shared__ unsigned char smem[];
     const BItem it = items[map[blockIdx.x]];

@@ -413,31 +413,304 @@ void ratio_table(Program& prog, FOp& op, const FPhase& ph) {
     }
 }
 
-// Moves every diagonal of the phase's ops back past the ops it commutes
-// with (matrix ops on register bits its tables do not depend on; any
-// diagonal) and multiplies it into an earlier diagonal where possible.
+// Diagonals commute with one another and with every matrix op whose target
+// register bits their tables do not depend on, so a diagonal may sit
+// anywhere between the last earlier and the first later matrix op that
+// targets one of its bits (its window of slots; slot s lies before matrix
+// op s).  In order of window end, each diagonal joins the first group
+// placed inside its window whose table it merges with, or opens a group at
+// its window's end; every group is one table.
 void merge_diagonals(Program& prog, uint32_t begin) {
-    std::vector<FOp> out;
+    std::vector<FOp> mats;
+    struct Diag { FOp op; uint32_t dep, lo, hi, at; };
+    std::vector<Diag> diags;
     for (size_t o = begin; o < prog.ops.size(); ++o) {
-        const FOp op = prog.ops[o];
+        const FOp& op = prog.ops[o];
+        if (op.type == OP_D) diags.push_back({op, diag_deps(prog, op), 0, 0, (uint32_t)mats.size()});
+        else mats.push_back(op);
+    }
+    const uint32_t m = (uint32_t)mats.size();
+    for (Diag& d : diags) {
+        d.lo = 0;
+        d.hi = m;
+        for (uint32_t k = 0; k < d.at; ++k)
+            if (reg_targets(mats[k]) & d.dep) d.lo = k + 1;
+        for (uint32_t k = m; k-- > d.at;)
+            if (reg_targets(mats[k]) & d.dep) d.hi = k;
+    }
+    std::vector<size_t> order(diags.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return diags[a].hi < diags[b].hi; });
+    struct Group { uint32_t slot; FOp op; };
+    std::vector<Group> groups;
+    for (size_t k : order) {
+        const Diag& d = diags[k];
         bool merged = false;
-        if (op.type == OP_D) {
-            const uint32_t dep = diag_deps(prog, op);
-            for (size_t j = out.size(); j-- > 0;) {
-                if (out[j].type == OP_D) {
-                    if (merge_diag(prog, out[j], op)) {
-                        merged = true;
-                        break;
-                    }
-                    continue;
-                }
-                if (reg_targets(out[j]) & dep) break;
+        for (Group& g : groups)
+            if (g.slot >= d.lo && g.slot <= d.hi && merge_diag(prog, g.op, d.op)) {
+                merged = true;
+                break;
             }
-        }
-        if (!merged) out.push_back(op);
+        if (!merged) groups.push_back({d.hi, d.op});
     }
     prog.ops.resize(begin);
-    prog.ops.insert(prog.ops.end(), out.begin(), out.end());
+    for (uint32_t s = 0; s <= m; ++s) {
+        for (const Group& g : groups)
+            if (g.slot == s) prog.ops.push_back(g.op);
+        if (s < m) prog.ops.push_back(mats[s]);
+    }
+}
+
+// A diagonal on register bit J of the phase, d0 / d1 where the bit is
+// 0 / 1, for the register indices in pc (phase 1 elsewhere); all four
+// variants alike.
+FOp diag_on(Program& prog, uint32_t J, uint32_t pc, cd d0, cd d1) {
+    FOp d{};
+    d.type = OP_D;
+    d.pc = 0xFFFFu;
+    cd tab[4 * kRegs];
+    for (int k = 0; k < 4; ++k)
+        for (int i = 0; i < kRegs; ++i)
+            tab[k * kRegs + i] = ((pc >> i) & 1u) ? ((i >> J) & 1 ? d1 : d0) : cd(1.0, 0.0);
+    d.coef = add_coef(prog, tab, 4 * kRegs);
+    return d;
+}
+
+// Splits the 2x2 ops of a phase whose controls are register bits only
+// (so the diagonals can fold them in):
+//   A2 [[0, a], [b, 0]] = X2 * diag(b, a);
+//   M2 = diag(l0, l1) R diag(1, r1) with R = [[c, -s], [s, c]] real,
+//   c = |m00|, s = |m10|, l0 = m00 / c, l1 = m10 / s, r1 = m11 / (l1 c),
+//   and R = c [[1, -t], [t, 1]] with t = s / c (T2), the scale c going
+//   into the left diagonal.  With FMA the T2 outputs x0 - t x1 and
+//   x1 + t x0 round once, so a large t (c small) loses nothing once scaled
+//   back by c.  An M2 that does not reproduce to 1e-14 (a non-unitary
+//   matrix, an overflowing t) or has a zero entry stays whole.
+void split_pairs(Program& prog, uint32_t begin) {
+    std::vector<FOp> in(prog.ops.begin() + begin, prog.ops.end());
+    prog.ops.resize(begin);
+    for (const FOp& op : in) {
+        if ((op.type != OP_M2 && op.type != OP_A2) || op.lctrl || op.gctrl) {
+            prog.ops.push_back(op);
+            continue;
+        }
+        auto cf = [&](int k) { return cd(prog.coef[op.coef + 2 * k], prog.coef[op.coef + 2 * k + 1]); };
+        if (op.type == OP_A2) {
+            const cd a = cf(0), b = cf(1);
+            prog.ops.push_back(diag_on(prog, op.sel, op.pc, b, a));
+            FOp x = op;
+            x.type = OP_X2;
+            x.coef = 0;
+            prog.ops.push_back(x);
+            continue;
+        }
+        const cd m00 = cf(0), m01 = cf(1), m10 = cf(2), m11 = cf(3);
+        const double c = std::abs(m00), s = std::abs(m10);
+        if (!(c > 0.0 && s > 0.0)) {
+            prog.ops.push_back(op);
+            continue;
+        }
+        const cd l0 = m00 / c, l1 = m10 / s, r1 = m11 / (l1 * c);
+        const double a = s / c;
+        // the product the kernel computes, entry by entry
+        const cd L0 = l0 * c, L1 = l1 * c;
+        const cd q00 = L0, q01 = -L0 * a * r1, q10 = L1 * a, q11 = L1 * r1;
+        const double err = std::max(std::max(std::abs(q00 - m00), std::abs(q01 - m01)),
+                                    std::max(std::abs(q10 - m10), std::abs(q11 - m11)));
+        if (!(err <= 1e-14)) {
+            prog.ops.push_back(op);
+            continue;
+        }
+        if (r1 != cd(1.0, 0.0)) prog.ops.push_back(diag_on(prog, op.sel, op.pc, cd(1.0, 0.0), r1));
+        FOp t = op;
+        t.type = OP_T2;
+        const cd av(a, 0.0);
+        t.coef = add_coef(prog, &av, 1);
+        prog.ops.push_back(t);
+        if (L0 != cd(1.0, 0.0) || L1 != cd(1.0, 0.0)) prog.ops.push_back(diag_on(prog, op.sel, op.pc, L0, L1));
+    }
+}
+
+// Estimated vector instructions per thread of one op: FP64 FMAs per
+// output amplitude times the outputs it writes, plus a fixed cost for the
+// op's decode (dependent scalar loads, branches) and for moving its
+// results back into the registers the op loop carries.
+constexpr uint32_t kOpOverhead = 40;
+uint32_t op_cost(const FOp& op) {
+    const uint32_t outs = (op.type == OP_D || op.type == OP_S) ? (uint32_t)kRegs : (uint32_t)popc(op.pc & 0xFFFFu);
+    uint32_t per = 0;
+    switch (op.type) {
+    case OP_D: per = has_ext(op) ? 6 : 4; break;
+    case OP_M2: per = 8; break;
+    case OP_M4: per = 16; break;
+    case OP_P4: per = 8; break;
+    case OP_A2: per = 4; break;
+    case OP_X2: per = 1; break;
+    case OP_G: per = 0; break;
+    case OP_S: per = 4 + 4 * 2 + 4; break;
+    default: per = 2; break;  // T2
+    }
+    return per * outs + kOpOverhead;
+}
+
+// Picks between a phase with and without split 2x2s.
+uint32_t phase_cost(const Program& prog, uint32_t begin) {
+    uint32_t cost = 0;
+    for (size_t o = begin; o < prog.ops.size(); ++o) cost += op_cost(prog.ops[o]);
+    return cost;
+}
+
+// A diagonal on register bits only, without controls outside the
+// registers: its variant-0 table is the whole op.
+bool plain_diag(const FOp& op) { return op.type == OP_D && !has_ext(op) && !op.lctrl && !op.gctrl; }
+
+// Folds runs [plain diagonal] [T2 on distinct register bits, no
+// controls] [plain diagonal] into one S op where the estimate says it is
+// cheaper (missing diagonals become identity tables, missing bits a = 0).
+void fold_structured(Program& prog, uint32_t begin) {
+    const std::vector<FOp> in(prog.ops.begin() + begin, prog.ops.end());
+    prog.ops.resize(begin);
+    const size_t n = in.size();
+    size_t i = 0;
+    while (i < n) {
+        size_t j = i;
+        const FOp* dr = nullptr;
+        const FOp* dl = nullptr;
+        if (plain_diag(in[j])) dr = &in[j++];
+        uint32_t bits = 0;
+        double t[kRegBits] = {0.0, 0.0, 0.0, 0.0};
+        while (j < n && in[j].type == OP_T2 && in[j].pc == 0xFFFFu && !in[j].lctrl && !in[j].gctrl &&
+               !((bits >> in[j].sel) & 1u)) {
+            bits |= 1u << in[j].sel;
+            t[in[j].sel] = prog.coef[in[j].coef];
+            ++j;
+        }
+        if (bits && j < n && plain_diag(in[j])) dl = &in[j++];
+        uint32_t separate = 0;
+        for (size_t k = i; k < j; ++k) separate += op_cost(in[k]);
+        FOp sop{};
+        sop.type = OP_S;
+        sop.pc = 0xFFFFu;
+        if (!bits || separate <= op_cost(sop)) {
+            prog.ops.push_back(in[i]);
+            ++i;
+            continue;
+        }
+        cd block[2 * kRegs + kRegBits];
+        for (int k = 0; k < kRegs; ++k) {
+            block[k] = dr ? cd(prog.coef[dr->coef + 2 * k], prog.coef[dr->coef + 2 * k + 1]) : cd(1.0, 0.0);
+            block[kRegs + kRegBits + k] = dl ? cd(prog.coef[dl->coef + 2 * k], prog.coef[dl->coef + 2 * k + 1]) : cd(1.0, 0.0);
+        }
+        for (int k = 0; k < kRegBits; ++k) block[kRegs + k] = cd(t[k], 0.0);
+        sop.coef = add_coef(prog, block, 2 * kRegs + kRegBits);
+        prog.ops.push_back(sop);
+        i = j;
+    }
+}
+
+// Finishes a register phase's ops [begin, end): diagonals that depend on
+// bits outside the tile only become G ops at the end (a per-tile scalar
+// commutes with every op of the tile); the other diagonals are merged, with
+// and without split 2x2s, whichever is cheaper; then the ratio tables.
+void finish_phase(Program& prog, uint32_t begin, const FPhase& ph) {
+    std::vector<FOp> scalars;
+    {
+        std::vector<FOp> rest;
+        for (size_t o = begin; o < prog.ops.size(); ++o) {
+            const FOp& op = prog.ops[o];
+            if (op.type == OP_D && !op.la && !op.lb && !op.lctrl && diag_deps(prog, op) == 0) {
+                FOp g = op;
+                g.type = OP_G;
+                scalars.push_back(g);
+            } else {
+                rest.push_back(op);
+            }
+        }
+        prog.ops.resize(begin);
+        prog.ops.insert(prog.ops.end(), rest.begin(), rest.end());
+    }
+    const std::vector<FOp> ops0(prog.ops.begin() + begin, prog.ops.end());
+    const std::vector<double> coef0 = prog.coef;
+    merge_diagonals(prog, begin);
+    const uint32_t whole = phase_cost(prog, begin);
+    const std::vector<FOp> opsW(prog.ops.begin() + begin, prog.ops.end());
+    const std::vector<double> coefW = prog.coef;
+    prog.ops.resize(begin);
+    prog.ops.insert(prog.ops.end(), ops0.begin(), ops0.end());
+    prog.coef = coef0;
+    split_pairs(prog, begin);
+    merge_diagonals(prog, begin);
+    fold_structured(prog, begin);
+    if (phase_cost(prog, begin) >= whole) {
+        prog.ops.resize(begin);
+        prog.ops.insert(prog.ops.end(), opsW.begin(), opsW.end());
+        prog.coef = coefW;
+    }
+    for (size_t o = begin; o < prog.ops.size(); ++o) ratio_table(prog, prog.ops[o], ph);
+    prog.ops.insert(prog.ops.end(), scalars.begin(), scalars.end());
+}
+
+struct PhasePlan {
+    uint32_t Rl;               // local positions of the phase's register bits
+    std::vector<Prim> prims;   // in application order
+};
+
+// How a diagonal would be applied in a phase: 0 every operand and control
+// is a register bit (the table merges with the phase's others), 1 its
+// non-register bits are uniform over a wave (outside the tile, or thread
+// bits >= 6), 2 each lane needs its own table or branch.
+int diag_score(const Prim& d, const PhasePlan& pl, const FPass& fp) {
+    int score = 0;
+    for (uint64_t m = d.bits(); m; m &= m - 1) {
+        const int l = local_of(fp.tpos, fp.t, (int)ctz(m));
+        if (l < 0) { score = std::max(score, 1); continue; }
+        if ((pl.Rl >> l) & 1u) continue;
+        const int tbit = l - (int)popc(pl.Rl & ((1u << l) - 1u));
+        score = std::max(score, tbit >= 6 ? 1 : 2);
+    }
+    return score;
+}
+
+// Moves each diagonal to the phase of the pass where it is cheapest
+// (diag_score), within the phases it can reach: it commutes with every
+// primitive except matrix primitives acting on one of its qubits, so it
+// may move to the start of any later phase up to the one holding the next
+// such primitive, or to the end of any earlier phase back to the one
+// holding the previous such primitive.  Ties keep it where it is.
+void relocate_diagonals(std::vector<PhasePlan>& plans, const FPass& fp) {
+    const int np = (int)plans.size();
+    for (int P = 0; P < np; ++P) {
+        for (size_t k = 0; k < plans[P].prims.size();) {
+            const Prim d = plans[P].prims[k];
+            if (d.type != OP_D) { ++k; continue; }
+            const uint64_t b = d.bits();
+            int hi = np - 1, lo = 0;
+            bool found = false;
+            for (size_t j = k + 1; j < plans[P].prims.size() && !found; ++j)
+                if (plans[P].prims[j].targets() & b) { hi = P; found = true; }
+            for (int Q = P + 1; Q < np && !found; ++Q)
+                for (const Prim& x : plans[Q].prims)
+                    if (x.targets() & b) { hi = Q; found = true; break; }
+            found = false;
+            for (size_t j = k; j-- > 0 && !found;)
+                if (plans[P].prims[j].targets() & b) { lo = P; found = true; }
+            for (int Q = P - 1; Q >= 0 && !found; --Q)
+                for (const Prim& x : plans[Q].prims)
+                    if (x.targets() & b) { lo = Q; found = true; break; }
+            int best = P, best_score = diag_score(d, plans[P], fp);
+            for (int Q = lo; Q <= hi; ++Q) {
+                const int sc = diag_score(d, plans[Q], fp);
+                if (sc < best_score || (sc == best_score && Q != best && std::abs(Q - P) < std::abs(best - P))) {
+                    best = Q;
+                    best_score = sc;
+                }
+            }
+            if (best == P) { ++k; continue; }
+            plans[P].prims.erase(plans[P].prims.begin() + (long)k);
+            if (best > P) plans[best].prims.insert(plans[best].prims.begin(), d);  // met again when P reaches it
+            else plans[best].prims.push_back(d);
+        }
+    }
 }
 
 void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
@@ -458,7 +731,7 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
 
     // phases: greedy register sets of four local qubits
     std::vector<Prim> rest = pp.prims;
-    fp.phase_begin = (uint32_t)prog.phases.size();
+    std::vector<PhasePlan> plans;
     while (!rest.empty()) {
         uint32_t Rl = 0;  // local register bits
         uint64_t blocked = 0;
@@ -476,7 +749,16 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
         for (int k = (int)fp.t - 1; k >= 0 && popc(Rl) < kRegBits; --k)
             if ((top_regs >> k) & 1) Rl |= 1u << k;
         for (int k = 0; k < (int)fp.t && popc(Rl) < kRegBits; ++k) Rl |= 1u << k;
+        plans.push_back({Rl, std::move(take)});
+        rest.swap(leave);
+    }
+    relocate_diagonals(plans, fp);
 
+    fp.phase_begin = (uint32_t)prog.phases.size();
+    for (PhasePlan& plan : plans) {
+        if (plan.prims.empty() && plans.size() > 1) continue;  // its diagonals all moved
+        const uint32_t Rl = plan.Rl;
+        const std::vector<Prim>& take = plan.prims;
         FPhase ph = make_phase(Rl);
         auto regbit = [&](int q) -> int {
             const int l = local_of(fp.tpos, fp.t, q);
@@ -571,10 +853,7 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
             if (op.lctrl || op.gctrl || op.la || op.lb || op.ga || op.gb) foldable = false;
             prog.ops.push_back(op);
         }
-        if (!foldable) {
-            merge_diagonals(prog, ph.op_begin);
-            for (size_t o = ph.op_begin; o < prog.ops.size(); ++o) ratio_table(prog, prog.ops[o], ph);
-        }
+        if (!foldable) finish_phase(prog, ph.op_begin, ph);
         ph.op_end = (uint32_t)prog.ops.size();
         if (foldable) {
             cd U[kRegs * kRegs];
@@ -586,7 +865,6 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
             ph.ucoef = add_coef(prog, U, kRegs * kRegs);
         }
         prog.phases.push_back(ph);
-        rest.swap(leave);
     }
     fp.phase_end = (uint32_t)prog.phases.size();
     auto is_top = [&](const FPhase& ph) {

@@ -15,11 +15,18 @@
 //      then commutes with everything it overtakes;
 //   4. split each pass into phases: a phase holds four of the tile's qubits
 //      in registers (16 amplitudes per thread); its matrix primitives act
-//      on those, diagonals on any qubit;
+//      on those, diagonals on any qubit; each diagonal then moves to the
+//      phase within its reach where its qubits are registers (or uniform
+//      over a wave);
 //   5. (dense_min > 0, tiles of >= 2^10 amplitudes) a phase of at least
 //      dense_min primitives that all act on its four register qubits only
 //      (operands and controls) becomes one dense 16x16 unitary, applied as
-//      a complex GEMM on the matrix cores.
+//      a complex GEMM on the matrix cores;
+//   6. other phases: 2x2s split into diagonal * real rotation * diagonal,
+//      diagonals merged within their windows, runs of diagonal / rotations
+//      / diagonal folded into one structured op, diagonals on bits outside
+//      the tile reduced to a per-tile scalar -- kept only where the op-cost
+//      estimate says the phase gets cheaper.
 #pragma once
 
 #include <cstdint>
@@ -45,14 +52,23 @@ constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane wa
 // M2: 2x2 on one register bit; M4: 4x4 on two; D: diagonal table;
 // P4: a 4x4 that only pairs index r with r ^ 3 (rxx, ryy, the swap family):
 // each output is two products instead of four; A2: an anti-diagonal 2x2
-// (y, or x with a phase); X2: the pure swap (x, cx).
-enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5 };
+// (y, or x with a phase); X2: the pure swap (x, cx); T2: the real middle
+// factor of a 2x2 split into diagonal * real * diagonal (see split_pairs
+// in fusion.cpp), one real coefficient a: out0 = x0 - a x1,
+// out1 = x1 + a x0, two FMAs per amplitude instead of the eight of a
+// complex 2x2; S: one structured block, a register diagonal, T2 on each of
+// the four register bits (a = 0, an exact identity, where a bit has none)
+// and a second register diagonal, in one op without branches; G: a
+// diagonal whose phase depends on bits outside the tile only (a scalar per
+// tile): the kernel multiplies the phase's G factors into one uniform
+// scalar and scales the amplitudes once, after the phase's other ops.
+enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5, OP_T2 = 6, OP_S = 7, OP_G = 8 };
 
 // All program fields are 32/64-bit and uniform across a block, so the
 // kernel reads them with scalar loads into SGPRs.
 struct FOp {                // 64 bytes
-    uint32_t type;          // OP_M2 / OP_M4 / OP_D
-    uint32_t sel;           // M2/A2/X2: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
+    uint32_t type;          // OP_*
+    uint32_t sel;           // M2/A2/X2/T2: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
     uint32_t pc;            // matrix ops: register indices that satisfy the register controls (0xFFFF: all)
     uint32_t lctrl;         // controls on lane bits (local positions)
     uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)
@@ -70,7 +86,11 @@ struct FOp {                // 64 bytes
 // (1 or 2) holds the ratio T_v / T_0, applied after T_0 by the waves whose
 // operand bit is set -- every table load is uniform.  dmode 0: each lane
 // reads its own variant's table.  M2: 4 complex; M4: 16 complex;
-// P4: (u[r][r], u[r][r^3]) for r = 0..3; A2: (u01, u10); X2: none.
+// P4: (u[r][r], u[r][r^3]) for r = 0..3; A2: (u01, u10); X2: none;
+// T2: one complex whose real part is a; S: 16 complex (first diagonal),
+// 4 complex whose real parts are the T2 coefficients of register bits
+// 0..3, 16 complex (second diagonal); G: a D table (entry 0 of variant k is
+// its factor).
 
 struct FPhase {
     uint32_t op_begin, op_end;

@@ -36,10 +36,16 @@ static C cf(const Program& P, uint32_t off, int k) {
 // One register phase of one thread: the ops of fused.hip's run_phase.
 static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uint64_t tile) {
     CHECK(ph.op_begin <= ph.op_end && ph.op_end <= P.ops.size());
+    C scal = 1.0;
     for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
         const FOp& op = P.ops[o];
         if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
-        if (op.type == OP_D) {
+        if (op.type == OP_G) {
+            CHECK(!op.la && !op.lb && !op.lctrl);
+            const uint32_t k = ((tile & op.ga) != 0 ? 1u : 0u) | ((tile & op.gb) != 0 ? 2u : 0u);
+            for (int i = 1; i < kRegs; ++i) CHECK(cf(P, op.coef, k * kRegs + i) == cf(P, op.coef, k * kRegs));
+            scal = cf(P, op.coef, k * kRegs) * scal;
+        } else if (op.type == OP_D) {
             const uint32_t k = ((((tb & op.la) | (tile & op.ga)) != 0) ? 1u : 0u) |
                                ((((tb & op.lb) | (tile & op.gb)) != 0) ? 2u : 0u);
             const bool ea = op.la || op.ga, eb = op.lb || op.gb;
@@ -52,7 +58,20 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
             } else {
                 for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, k * kRegs + i) * v[i];
             }
-        } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2) {
+        } else if (op.type == OP_S) {
+            for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
+            for (int j = 0; j < kRegBits; ++j) {
+                CHECK(cf(P, op.coef, kRegs + j).imag() == 0.0);
+                const double a = cf(P, op.coef, kRegs + j).real();
+                for (int i = 0; i < kRegs; ++i) {
+                    if (i & (1 << j)) continue;
+                    const C x0 = v[i], x1 = v[i | (1 << j)];
+                    v[i] = x0 - a * x1;
+                    v[i | (1 << j)] = x1 + a * x0;
+                }
+            }
+            for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, kRegs + kRegBits + i) * v[i];
+        } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2 || op.type == OP_T2) {
             const int J = (int)op.sel;
             CHECK(J < kRegBits);
             for (int i = 0; i < kRegs; ++i) {
@@ -65,6 +84,11 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                 } else if (op.type == OP_A2) {
                     v[i] = cf(P, op.coef, 0) * x1;
                     v[i | (1 << J)] = cf(P, op.coef, 1) * x0;
+                } else if (op.type == OP_T2) {
+                    const double a = cf(P, op.coef, 0).real();
+                    CHECK(cf(P, op.coef, 0).imag() == 0.0);
+                    v[i] = x0 - a * x1;
+                    v[i | (1 << J)] = x1 + a * x0;
                 } else {
                     v[i] = x1;
                     v[i | (1 << J)] = x0;
@@ -92,6 +116,7 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
             }
         }
     }
+    for (int i = 0; i < kRegs; ++i) v[i] = scal * v[i];
 }
 
 // One pass over the state: every tile in turn (fused.hip: fused_tile).
@@ -229,9 +254,33 @@ static std::vector<qvnt_gate> random_circuit(std::mt19937_64& g, unsigned n, int
     return out;
 }
 
-static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed) {
+// Layers of u3 on every qubit, rzz on neighbours (a ring, so the last
+// edge wraps) and one controlled x: every phase of such a pass is a
+// structured block, and the diagonals cross tiles and phases.
+static std::vector<qvnt_gate> layered_circuit(std::mt19937_64& g, unsigned n, int layers) {
+    std::vector<qvnt_gate> out;
+    std::uniform_real_distribution<double> ang(0.0, 6.283185307179586);
+    qvnt_gate buf[64];
+    for (int l = 0; l < layers; ++l) {
+        for (unsigned q = 0; q < n; ++q) {
+            const long m = orc_op_u3(ang(g), ang(g), ang(g), 1ull << q, buf);
+            out.insert(out.end(), buf, buf + m);
+        }
+        for (unsigned q = l & 1; q < n; q += 2) {
+            const long m = orc_op_rzz(ang(g), (1ull << q) | (1ull << ((q + 1) % n)), buf);
+            out.insert(out.end(), buf, buf + m);
+        }
+        const unsigned c = (unsigned)(g() % n), t = (c + 1 + (unsigned)(g() % (n - 1))) % n;
+        long m = orc_op_x(1ull << t, buf);
+        CHECK(orc_op_c(buf, (size_t)m, 1ull << c) == 0);
+        out.insert(out.end(), buf, buf + m);
+    }
+    return out;
+}
+
+static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false) {
     std::mt19937_64 g(seed);
-    const std::vector<qvnt_gate> gates = random_circuit(g, n, 160);
+    const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
     plan(gates.data(), gates.size(), n, tq, steps, dense);
     orc_reg* ref = orc_reg_with_state(n, 0);
@@ -241,7 +290,7 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed) {
     for (auto& z : psi) z = C(nd(g), nd(g));
     std::memcpy(orc_reg_psi(ref), psi.data(), len * sizeof(C));
     orc_reg_apply(ref, gates.data(), gates.size());
-    unsigned passes = 0, dense_phases = 0;
+    unsigned passes = 0, dense_phases = 0, split = 0;
     for (const Step& st : steps) {
         if (st.plain) {
             orc_reg* one = orc_reg_with_state(n, 0);  // a gate the fused kernel does not take
@@ -252,6 +301,7 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed) {
             continue;
         }
         for (const FPhase& ph : st.prog.phases) dense_phases += ph.dense;
+        for (const FOp& op : st.prog.ops) split += op.type == OP_T2 || op.type == OP_S;
         for (const FPass& fp : st.prog.passes) {
             run_pass(psi.data(), len, st.prog, fp);
             ++passes;
@@ -260,8 +310,8 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed) {
     const C* want = reinterpret_cast<const C*>(orc_reg_psi(ref));
     double err = 0.0;
     for (uint64_t i = 0; i < len; ++i) err = std::max(err, std::abs(psi[i] - want[i]));
-    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u dense phases, max |d| %.2e\n", n, tq, dense,
-                gates.size(), passes, dense_phases, err);
+    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u dense phases, %u split/structured ops, max |d| %.2e\n", n, tq,
+                dense, gates.size(), passes, dense_phases, split, err);
     if (!(err <= 1e-12)) ++failures;
     orc_reg_free(ref);
 }
@@ -271,6 +321,8 @@ int main() {
                                  {13, 12, 2}, {13, 13, 0}, {14, 11, 0}, {14, 13, 3}, {15, 12, 2}};
     uint64_t seed = 1;
     for (const auto& c : cases) check_case(c[0], c[1], c[2], seed++);
+    const unsigned layered[][3] = {{10, 10, 0}, {13, 12, 0}, {14, 11, 0}, {15, 13, 0}, {16, 12, 3}};
+    for (const auto& c : layered) check_case(c[0], c[1], c[2], seed++, true);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
