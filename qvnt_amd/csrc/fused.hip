@@ -166,6 +166,15 @@ __device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c
     for (int i = 0; i < kRegs; ++i) v[i] = mul_c(ld_coef<R>(c, i), v[i]);
 }
 
+// a phase on the amplitudes whose register bit J is 1 (fusion.hpp: OP_PH)
+template <int J, class R>
+__device__ __forceinline__ void apply_ph(cplx<R>* v, const double* __restrict__ c) {
+    const cplx<R> g = ld_coef<R>(c, 0);
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i)
+        if (i & (1 << J)) v[i] = mul_c(g, v[i]);
+}
+
 // the structured block (fusion.hpp: OP_S): diagonal, T2 on register bits
 // 0..3, diagonal -- straight-line code, so the 16 amplitudes stay in
 // registers from the first product to the last
@@ -214,6 +223,13 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
             }
         } else if (op.type == OP_S) {
             apply_s(v, c);
+        } else if (op.type == OP_PH) {
+            switch (op.sel) {
+            case 0: apply_ph<0>(v, c); break;
+            case 1: apply_ph<1>(v, c); break;
+            case 2: apply_ph<2>(v, c); break;
+            default: apply_ph<3>(v, c); break;
+            }
         } else if (op.type == OP_T2) {
             apply_t2_sel(v, c, op.sel, op.pc);
         } else if (op.type == OP_M2) {

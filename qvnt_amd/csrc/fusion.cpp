@@ -55,6 +55,19 @@ Prim diag2(int q0, int q1, cd t0, cd t1, cd t2, cd t3, uint64_t ctrl) {
     p.m[0] = t0; p.m[1] = t1; p.m[2] = t2; p.m[3] = t3;
     return p;
 }
+// A two-qubit phase table f(a, b) (t[a | 2b]) as three factors that
+// commute: f = t0 * (t1/t0)^a * (t2/t0)^b * gamma^(ab), gamma = t3 t0 /
+// (t1 t2) -- a phase on qubit a, one on qubit b, and a phase on |11>
+// (qubit a's phase controlled by b, dropped when it is 1).  The one-qubit
+// factors can go to whichever phase holds their qubit in registers; only
+// the |11> phase needs both qubits.
+void push_diag2(std::vector<Prim>& out, int q0, int q1, cd t0, cd t1, cd t2, cd t3, uint64_t ctrl) {
+    out.push_back(diag1(q0, t0, t1, ctrl));
+    out.push_back(diag1(q1, 1.0, t2 / t0, ctrl));
+    const cd g = t3 * t0 / (t1 * t2);
+    if (g != cd(1.0, 0.0)) out.push_back(diag1(q0, 1.0, g, ctrl | (1ull << q1)));
+}
+
 Prim m4(int q0, int q1, const cd* u, uint64_t ctrl) {
     Prim p;
     p.type = OP_M4;
@@ -122,7 +135,7 @@ bool lower(const qvnt_gate& g, uint64_t lmask, std::vector<Prim>& out) {
         const std::vector<int> b = bitlist(g.a);
         const std::vector<int> in = bitlist(g.a & lmask);
         if (in.size() == 2) {
-            out.push_back(diag2(b[0], b[1], std::conj(e), e, e, std::conj(e), ctrl));
+            push_diag2(out, b[0], b[1], std::conj(e), e, e, std::conj(e), ctrl);
         } else if (in.size() == 1) {
             out.push_back(diag1(in[0], std::conj(e), e, ctrl));
         } else {
@@ -222,24 +235,55 @@ struct PassPlan {
     std::vector<Prim> prims;
 };
 
-// Greedy pass formation (see the file comment).
+// The primitives left behind so far in a scan of the queue.  A later
+// primitive may overtake them only if it commutes with each: diagonals
+// commute with one another, a diagonal and a matrix primitive commute
+// when the matrix targets none of the diagonal's qubits, and two matrix
+// primitives are taken to commute only on disjoint qubits.
+struct Left {
+    uint64_t diag_bits = 0, mat_bits = 0, mat_targets = 0;
+    bool blocks(const Prim& p) const {
+        if (p.type == OP_D) return (p.bits() & mat_targets) != 0;
+        return (p.targets() & diag_bits) != 0 || (p.bits() & mat_bits) != 0;
+    }
+    void add(const Prim& p) {
+        if (p.type == OP_D) diag_bits |= p.bits();
+        else { mat_bits |= p.bits(); mat_targets |= p.targets(); }
+    }
+};
+
+// Greedy pass formation (see the file comment).  A second scan with the
+// pass's final qubit set T leaves for a later pass every diagonal that
+// touches no qubit of T (in this pass it would only scale whole tiles; in
+// a later one its qubits may be registers), provided the pass keeps some
+// matrix primitive, so every pass makes progress.
 std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsigned tq) {
     std::vector<PassPlan> passes;
     const uint64_t forced = (ns >= 3) ? 0x7u : ((1ull << ns) - 1u);
     while (!rest.empty()) {
         PassPlan pp;
         pp.T = forced;
-        uint64_t blocked = 0;
+        Left left;
+        bool matrix = false;
         std::vector<Prim> leave;
         for (const Prim& p : rest) {
-            const uint64_t b = p.bits();
-            if (b & blocked) { blocked |= b; leave.push_back(p); continue; }
+            if (left.blocks(p)) { left.add(p); leave.push_back(p); continue; }
             const uint64_t nt = pp.T | p.targets();
-            if (popc(nt) <= tq) { pp.T = nt; pp.prims.push_back(p); }
-            else { blocked |= b; leave.push_back(p); }
+            if (popc(nt) <= tq) { pp.T = nt; pp.prims.push_back(p); matrix |= p.type != OP_D; }
+            else { left.add(p); leave.push_back(p); }
         }
         // pad T with the lowest free qubits up to tq (larger, fewer tiles)
         for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
+        if (matrix) {
+            pp.prims.clear();
+            leave.clear();
+            left = Left();
+            for (const Prim& p : rest) {
+                const bool defer = p.type == OP_D && !(p.bits() & pp.T);
+                if (left.blocks(p) || defer || (p.targets() & ~pp.T)) { left.add(p); leave.push_back(p); continue; }
+                pp.prims.push_back(p);
+            }
+        }
         passes.push_back(std::move(pp));
         rest.swap(leave);
     }
@@ -354,6 +398,10 @@ uint32_t diag_deps(const Program& prog, const FOp& op) {
 
 bool has_ext(const FOp& op) { return op.la || op.lb || op.ga || op.gb; }
 
+// A diagonal on register bits only, without controls outside the
+// registers: its variant-0 table is the whole op.
+bool plain_diag(const FOp& op) { return op.type == OP_D && !has_ext(op) && !op.lctrl && !op.gctrl; }
+
 // Merges diagonal `d` (later) into diagonal `e` (earlier) if their controls
 // and non-register operands allow one table set; true on success.
 bool merge_diag(Program& prog, FOp& e, const FOp& d) {
@@ -420,7 +468,7 @@ void ratio_table(Program& prog, FOp& op, const FPhase& ph) {
 // op s).  In order of window end, each diagonal joins the first group
 // placed inside its window whose table it merges with, or opens a group at
 // its window's end; every group is one table.
-void merge_diagonals(Program& prog, uint32_t begin) {
+void merge_diagonals(Program& prog, uint32_t begin, bool separate_ext = false) {
     std::vector<FOp> mats;
     struct Diag { FOp op; uint32_t dep, lo, hi, at; };
     std::vector<Diag> diags;
@@ -447,16 +495,29 @@ void merge_diagonals(Program& prog, uint32_t begin) {
         const Diag& d = diags[k];
         bool merged = false;
         for (Group& g : groups)
-            if (g.slot >= d.lo && g.slot <= d.hi && merge_diag(prog, g.op, d.op)) {
+            if (g.slot >= d.lo && g.slot <= d.hi && (!separate_ext || plain_diag(g.op) == plain_diag(d.op)) &&
+                merge_diag(prog, g.op, d.op)) {
                 merged = true;
                 break;
             }
-        if (!merged) groups.push_back({d.hi, d.op});
+        if (!merged) {
+            // kept apart, a table with non-register operands goes to the
+            // phase's start or end where it can, out of the matrix ops' run
+            uint32_t slot = d.hi;
+            if (separate_ext && !plain_diag(d.op)) slot = d.lo == 0 ? 0 : d.hi;
+            groups.push_back({slot, d.op});
+        }
     }
+    // within a slot, tables on register bits only sit next to the matrix
+    // ops (before the next one, or after the last), where a structured op
+    // can take them in
     prog.ops.resize(begin);
     for (uint32_t s = 0; s <= m; ++s) {
-        for (const Group& g : groups)
-            if (g.slot == s) prog.ops.push_back(g.op);
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool plain_now = (s == m) == (pass == 0);
+            for (const Group& g : groups)
+                if (g.slot == s && plain_diag(g.op) == plain_now) prog.ops.push_back(g.op);
+        }
         if (s < m) prog.ops.push_back(mats[s]);
     }
 }
@@ -540,7 +601,7 @@ uint32_t op_cost(const FOp& op) {
     const uint32_t outs = (op.type == OP_D || op.type == OP_S) ? (uint32_t)kRegs : (uint32_t)popc(op.pc & 0xFFFFu);
     uint32_t per = 0;
     switch (op.type) {
-    case OP_D: per = has_ext(op) ? 6 : 4; break;
+    case OP_D: per = has_ext(op) ? 8 : 4; break;  // a table per variant: per-lane loads or a second product
     case OP_M2: per = 8; break;
     case OP_M4: per = 16; break;
     case OP_P4: per = 8; break;
@@ -548,6 +609,7 @@ uint32_t op_cost(const FOp& op) {
     case OP_X2: per = 1; break;
     case OP_G: per = 0; break;
     case OP_S: per = 4 + 4 * 2 + 4; break;
+    case OP_PH: return 4 * kRegs / 2 + kOpOverhead;
     default: per = 2; break;  // T2
     }
     return per * outs + kOpOverhead;
@@ -559,10 +621,6 @@ uint32_t phase_cost(const Program& prog, uint32_t begin) {
     for (size_t o = begin; o < prog.ops.size(); ++o) cost += op_cost(prog.ops[o]);
     return cost;
 }
-
-// A diagonal on register bits only, without controls outside the
-// registers: its variant-0 table is the whole op.
-bool plain_diag(const FOp& op) { return op.type == OP_D && !has_ext(op) && !op.lctrl && !op.gctrl; }
 
 // Folds runs [plain diagonal] [T2 on distinct register bits, no
 // controls] [plain diagonal] into one S op where the estimate says it is
@@ -608,6 +666,27 @@ void fold_structured(Program& prog, uint32_t begin) {
     }
 }
 
+// A diagonal on register bits whose table is 1 where register bit j is 0
+// and one value g where it is 1 becomes PH (8 products instead of 16).
+void phase_on_bit(Program& prog, FOp& op) {
+    if (op.type != OP_D || has_ext(op)) return;
+    const double* t = prog.coef.data() + op.coef;
+    for (uint32_t j = 0; j < (uint32_t)kRegBits; ++j) {
+        const int i1 = 1 << j;
+        bool ok = !(t[2 * i1] == 1.0 && t[2 * i1 + 1] == 0.0);
+        for (int i = 0; i < kRegs && ok; ++i) {
+            const double* e = t + 2 * i;
+            ok = (i >> j) & 1 ? (e[0] == t[2 * i1] && e[1] == t[2 * i1 + 1]) : (e[0] == 1.0 && e[1] == 0.0);
+        }
+        if (!ok) continue;
+        const cd g(t[2 * i1], t[2 * i1 + 1]);
+        op.type = OP_PH;
+        op.sel = j;
+        op.coef = add_coef(prog, &g, 1);
+        return;
+    }
+}
+
 // Finishes a register phase's ops [begin, end): diagonals that depend on
 // bits outside the tile only become G ops at the end (a per-tile scalar
 // commutes with every op of the tile); the other diagonals are merged, with
@@ -629,24 +708,35 @@ void finish_phase(Program& prog, uint32_t begin, const FPhase& ph) {
         prog.ops.resize(begin);
         prog.ops.insert(prog.ops.end(), rest.begin(), rest.end());
     }
+    // candidates: 2x2s whole or split; diagonals with non-register
+    // operands merged into the plain ones or kept apart (so the plain ones
+    // can join a structured op)
     const std::vector<FOp> ops0(prog.ops.begin() + begin, prog.ops.end());
     const std::vector<double> coef0 = prog.coef;
-    merge_diagonals(prog, begin);
-    const uint32_t whole = phase_cost(prog, begin);
-    const std::vector<FOp> opsW(prog.ops.begin() + begin, prog.ops.end());
-    const std::vector<double> coefW = prog.coef;
-    prog.ops.resize(begin);
-    prog.ops.insert(prog.ops.end(), ops0.begin(), ops0.end());
-    prog.coef = coef0;
-    split_pairs(prog, begin);
-    merge_diagonals(prog, begin);
-    fold_structured(prog, begin);
-    if (phase_cost(prog, begin) >= whole) {
+    std::vector<FOp> best_ops;
+    std::vector<double> best_coef;
+    uint32_t best = 0xFFFFFFFFu;
+    for (int variant = 0; variant < 3; ++variant) {
         prog.ops.resize(begin);
-        prog.ops.insert(prog.ops.end(), opsW.begin(), opsW.end());
-        prog.coef = coefW;
+        prog.ops.insert(prog.ops.end(), ops0.begin(), ops0.end());
+        prog.coef = coef0;
+        if (variant > 0) split_pairs(prog, begin);
+        merge_diagonals(prog, begin, variant == 2);
+        if (variant > 0) fold_structured(prog, begin);
+        const uint32_t cost = phase_cost(prog, begin);
+        if (cost < best) {
+            best = cost;
+            best_ops.assign(prog.ops.begin() + begin, prog.ops.end());
+            best_coef = prog.coef;
+        }
     }
-    for (size_t o = begin; o < prog.ops.size(); ++o) ratio_table(prog, prog.ops[o], ph);
+    prog.ops.resize(begin);
+    prog.ops.insert(prog.ops.end(), best_ops.begin(), best_ops.end());
+    prog.coef = best_coef;
+    for (size_t o = begin; o < prog.ops.size(); ++o) {
+        ratio_table(prog, prog.ops[o], ph);
+        phase_on_bit(prog, prog.ops[o]);
+    }
     prog.ops.insert(prog.ops.end(), scalars.begin(), scalars.end());
 }
 
@@ -655,18 +745,19 @@ struct PhasePlan {
     std::vector<Prim> prims;   // in application order
 };
 
-// How a diagonal would be applied in a phase: 0 every operand and control
-// is a register bit (the table merges with the phase's others), 1 its
-// non-register bits are uniform over a wave (outside the tile, or thread
-// bits >= 6), 2 each lane needs its own table or branch.
+// How a diagonal would be applied in a phase, summed over its operands and
+// controls: 0 for a register bit (the table merges with the phase's
+// others), 1 for a bit uniform over a wave (outside the tile, or a thread
+// bit >= 6), 3 for a bit that differs between the lanes of a wave (each
+// lane needs its own table or branch).
 int diag_score(const Prim& d, const PhasePlan& pl, const FPass& fp) {
     int score = 0;
     for (uint64_t m = d.bits(); m; m &= m - 1) {
         const int l = local_of(fp.tpos, fp.t, (int)ctz(m));
-        if (l < 0) { score = std::max(score, 1); continue; }
+        if (l < 0) { score += 1; continue; }
         if ((pl.Rl >> l) & 1u) continue;
         const int tbit = l - (int)popc(pl.Rl & ((1u << l) - 1u));
-        score = std::max(score, tbit >= 6 ? 1 : 2);
+        score += tbit >= 6 ? 1 : 3;
     }
     return score;
 }
@@ -734,16 +825,15 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     std::vector<PhasePlan> plans;
     while (!rest.empty()) {
         uint32_t Rl = 0;  // local register bits
-        uint64_t blocked = 0;
+        Left left;
         std::vector<Prim> take, leave;
         for (const Prim& p : rest) {
-            const uint64_t b = p.bits();
-            if (b & blocked) { blocked |= b; leave.push_back(p); continue; }
+            if (left.blocks(p)) { left.add(p); leave.push_back(p); continue; }
             if (p.type == OP_D) { take.push_back(p); continue; }
             uint32_t need = 0;
             for (uint64_t m = p.targets(); m; m &= m - 1) need |= 1u << local_of(fp.tpos, fp.t, (int)ctz(m));
             if (popc(Rl | need) <= kRegBits) { Rl |= need; take.push_back(p); }
-            else { blocked |= b; leave.push_back(p); }
+            else { left.add(p); leave.push_back(p); }
         }
         // complete the register set, preferring the top local bits (direct HBM access)
         for (int k = (int)fp.t - 1; k >= 0 && popc(Rl) < kRegBits; --k)
@@ -770,7 +860,20 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
         ph.op_begin = (uint32_t)prog.ops.size();
         const size_t coef_begin = prog.coef.size();
         bool foldable = dense_min > 0 && fp.t >= (uint32_t)kMinDenseTile && take.size() >= dense_min;
-        for (const Prim& p : take) {
+        for (const Prim& p0 : take) {
+            Prim p = p0;
+            // a controlled phase diag(1, g) on q0 is symmetric in q0 and its
+            // controls: make a register bit the operand where one is
+            if (p.type == OP_D && p.q1 < 0 && p.m[0] == cd(1.0, 0.0) && p.m[2] == cd(1.0, 0.0) && regbit(p.q0) < 0) {
+                for (uint64_t m = p.ctrl; m; m &= m - 1) {
+                    const int c = (int)ctz(m);
+                    if (regbit(c) >= 0) {
+                        p.ctrl = (p.ctrl & ~(1ull << c)) | (1ull << p.q0);
+                        p.q0 = c;
+                        break;
+                    }
+                }
+            }
             FOp op{};
             op.type = p.type;
             // controls: register / lane / outside-tile

@@ -61,14 +61,17 @@ constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane wa
 // and a second register diagonal, in one op without branches; G: a
 // diagonal whose phase depends on bits outside the tile only (a scalar per
 // tile): the kernel multiplies the phase's G factors into one uniform
-// scalar and scales the amplitudes once, after the phase's other ops.
-enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5, OP_T2 = 6, OP_S = 7, OP_G = 8 };
+// scalar and scales the amplitudes once, after the phase's other ops;
+// PH: a phase on the 8 amplitudes whose register bit j is 1 (the |11>
+// factor of a two-qubit diagonal, its other qubit a control).
+enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5, OP_T2 = 6, OP_S = 7, OP_G = 8,
+                 OP_PH = 9 };
 
 // All program fields are 32/64-bit and uniform across a block, so the
 // kernel reads them with scalar loads into SGPRs.
 struct FOp {                // 64 bytes
     uint32_t type;          // OP_*
-    uint32_t sel;           // M2/A2/X2/T2: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
+    uint32_t sel;           // M2/A2/X2/T2/PH: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
     uint32_t pc;            // matrix ops: register indices that satisfy the register controls (0xFFFF: all)
     uint32_t lctrl;         // controls on lane bits (local positions)
     uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)
@@ -90,7 +93,7 @@ struct FOp {                // 64 bytes
 // T2: one complex whose real part is a; S: 16 complex (first diagonal),
 // 4 complex whose real parts are the T2 coefficients of register bits
 // 0..3, 16 complex (second diagonal); G: a D table (entry 0 of variant k is
-// its factor).
+// its factor); PH: one complex.
 
 struct FPhase {
     uint32_t op_begin, op_end;

@@ -58,6 +58,10 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
             } else {
                 for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, k * kRegs + i) * v[i];
             }
+        } else if (op.type == OP_PH) {
+            CHECK(op.sel < (uint32_t)kRegBits && !op.la && !op.lb && !op.ga && !op.gb);
+            for (int i = 0; i < kRegs; ++i)
+                if ((i >> op.sel) & 1) v[i] = cf(P, op.coef, 0) * v[i];
         } else if (op.type == OP_S) {
             for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
             for (int j = 0; j < kRegBits; ++j) {
