@@ -275,11 +275,17 @@ std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsig
         // pad T with the lowest free qubits up to tq (larger, fewer tiles)
         for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
         if (matrix) {
+            // later matrix targets: a deferred diagonal must meet one of
+            // them, so some later pass holds its qubits anyway
+            std::vector<uint64_t> later(rest.size() + 1, 0);
+            for (size_t i = rest.size(); i-- > 0;)
+                later[i] = later[i + 1] | (i + 1 < rest.size() && rest[i + 1].type != OP_D ? rest[i + 1].targets() : 0);
             pp.prims.clear();
             leave.clear();
             left = Left();
-            for (const Prim& p : rest) {
-                const bool defer = p.type == OP_D && !(p.bits() & pp.T);
+            for (size_t i = 0; i < rest.size(); ++i) {
+                const Prim& p = rest[i];
+                const bool defer = p.type == OP_D && !(p.bits() & pp.T) && (p.bits() & later[i]);
                 if (left.blocks(p) || defer || (p.targets() & ~pp.T)) { left.add(p); leave.push_back(p); continue; }
                 pp.prims.push_back(p);
             }
@@ -666,6 +672,22 @@ void fold_structured(Program& prog, uint32_t begin) {
     }
 }
 
+// Complex coefficients an op reads (fused.hip): a diagonal without
+// operands outside the registers reads only variant 0 of its table.
+uint32_t coef_complex(const FOp& op) {
+    switch (op.type) {
+    case OP_D: return has_ext(op) ? 4 * kRegs : kRegs;
+    case OP_G: return 4 * kRegs;
+    case OP_M2: return 4;
+    case OP_M4: return 16;
+    case OP_P4: return 8;
+    case OP_A2: return 2;
+    case OP_X2: return 0;
+    case OP_S: return 2 * kRegs + kRegBits;
+    default: return 1;  // T2, PH
+    }
+}
+
 // A diagonal on register bits whose table is 1 where register bit j is 0
 // and one value g where it is 1 becomes PH (8 products instead of 16).
 void phase_on_bit(Program& prog, FOp& op) {
@@ -691,7 +713,7 @@ void phase_on_bit(Program& prog, FOp& op) {
 // bits outside the tile only become G ops at the end (a per-tile scalar
 // commutes with every op of the tile); the other diagonals are merged, with
 // and without split 2x2s, whichever is cheaper; then the ratio tables.
-void finish_phase(Program& prog, uint32_t begin, const FPhase& ph) {
+void finish_phase(Program& prog, uint32_t begin, size_t coef_begin, const FPhase& ph) {
     std::vector<FOp> scalars;
     {
         std::vector<FOp> rest;
@@ -711,15 +733,17 @@ void finish_phase(Program& prog, uint32_t begin, const FPhase& ph) {
     // candidates: 2x2s whole or split; diagonals with non-register
     // operands merged into the plain ones or kept apart (so the plain ones
     // can join a structured op)
+    // (every table these steps touch or add lies at or after coef_begin)
     const std::vector<FOp> ops0(prog.ops.begin() + begin, prog.ops.end());
-    const std::vector<double> coef0 = prog.coef;
+    const std::vector<double> coef0(prog.coef.begin() + (long)coef_begin, prog.coef.end());
     std::vector<FOp> best_ops;
     std::vector<double> best_coef;
     uint32_t best = 0xFFFFFFFFu;
     for (int variant = 0; variant < 3; ++variant) {
         prog.ops.resize(begin);
         prog.ops.insert(prog.ops.end(), ops0.begin(), ops0.end());
-        prog.coef = coef0;
+        prog.coef.resize(coef_begin);
+        prog.coef.insert(prog.coef.end(), coef0.begin(), coef0.end());
         if (variant > 0) split_pairs(prog, begin);
         merge_diagonals(prog, begin, variant == 2);
         if (variant > 0) fold_structured(prog, begin);
@@ -727,17 +751,31 @@ void finish_phase(Program& prog, uint32_t begin, const FPhase& ph) {
         if (cost < best) {
             best = cost;
             best_ops.assign(prog.ops.begin() + begin, prog.ops.end());
-            best_coef = prog.coef;
+            best_coef.assign(prog.coef.begin() + (long)coef_begin, prog.coef.end());
         }
     }
     prog.ops.resize(begin);
     prog.ops.insert(prog.ops.end(), best_ops.begin(), best_ops.end());
-    prog.coef = best_coef;
+    prog.coef.resize(coef_begin);
+    prog.coef.insert(prog.coef.end(), best_coef.begin(), best_coef.end());
     for (size_t o = begin; o < prog.ops.size(); ++o) {
         ratio_table(prog, prog.ops[o], ph);
         phase_on_bit(prog, prog.ops[o]);
     }
     prog.ops.insert(prog.ops.end(), scalars.begin(), scalars.end());
+    // keep only the coefficients the phase's ops read (merged, folded and
+    // rewritten tables leave unused blocks behind)
+    std::vector<double> kept;
+    for (size_t o = begin; o < prog.ops.size(); ++o) {
+        FOp& op = prog.ops[o];
+        const uint32_t n = 2 * coef_complex(op);
+        if (n == 0) { op.coef = 0; continue; }
+        const double* src = prog.coef.data() + op.coef;
+        op.coef = (uint32_t)(coef_begin + kept.size());
+        kept.insert(kept.end(), src, src + n);
+    }
+    prog.coef.resize(coef_begin);
+    prog.coef.insert(prog.coef.end(), kept.begin(), kept.end());
 }
 
 struct PhasePlan {
@@ -956,7 +994,7 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
             if (op.lctrl || op.gctrl || op.la || op.lb || op.ga || op.gb) foldable = false;
             prog.ops.push_back(op);
         }
-        if (!foldable) finish_phase(prog, ph.op_begin, ph);
+        if (!foldable) finish_phase(prog, ph.op_begin, coef_begin, ph);
         ph.op_end = (uint32_t)prog.ops.size();
         if (foldable) {
             cd U[kRegs * kRegs];
