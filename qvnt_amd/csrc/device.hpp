@@ -44,16 +44,19 @@ __device__ __forceinline__ unsigned dpopc(uint64_t x) { return (unsigned)__popcl
 // Bit positions inserted into a compact thread index to enumerate one
 // representative per orbit: `pos` ascending, then `set` ORed in (control
 // bits, and a fixed 1 where the orbit needs one).
+// Fixed bits of a launch: `zeros` holds their positions, `set` the ones
+// among them that are 1, n = popcount(zeros).  The walk over `zeros` is
+// uniform scalar work (no table loads in the kernels' address path).
 struct Deposit {
+    uint64_t zeros;
+    uint64_t set;
     uint32_t n;
     uint32_t pad;
-    uint64_t set;
-    uint8_t pos[64];
 };
 
 __device__ __forceinline__ uint64_t deposit(uint64_t g, const Deposit& d) {
-    for (uint32_t k = 0; k < d.n; ++k) {
-        const uint64_t lo = g & ((1ull << d.pos[k]) - 1u);
+    for (uint64_t m = d.zeros; m; m &= m - 1) {
+        const uint64_t lo = g & ((m & (0 - m)) - 1u);  // bits below the lowest position left
         g = ((g ^ lo) << 1) | lo;
     }
     return g | d.set;

@@ -352,12 +352,8 @@ int collapse(qvnt_reg* r, uint64_t outcome, uint64_t mask, bool renorm) {
     if (!(kept > 0.0) || !std::isfinite(kept)) return fail(QVNT_E_ZERO_NORM, "collapsed state has zero norm");
     const double scale = 1.0 / std::sqrt(kept);
     Deposit d{};
-    for (uint64_t rest = mask; rest; rest &= rest - 1) {
-        const unsigned p = ctz(rest);
-        unsigned k = d.n++;
-        while (k > 0 && d.pos[k - 1] > p) { d.pos[k] = d.pos[k - 1]; --k; }
-        d.pos[k] = (uint8_t)p;
-    }
+    d.zeros = mask;
+    d.n = popc(mask);
     d.set = outcome & mask;
     const uint64_t groups = r->len >> d.n;
     const uint64_t nb = (groups + kBlock - 1) / kBlock;

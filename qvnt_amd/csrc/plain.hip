@@ -153,7 +153,7 @@ int unroll_setting() {
     static int u = [] {
         const char* s = std::getenv("QVNT_UNROLL");
         int v = s ? std::atoi(s) : 2;
-        return (v == 1 || v == 2) ? v : 2;
+        return (v == 1 || v == 2 || v == 4) ? v : 2;
     }();
     return u;
 }
@@ -166,16 +166,9 @@ unsigned block_setting() {
     return b;
 }
 
-void add_pos(Deposit& d, unsigned p) {
-    unsigned k = d.n++;
-    while (k > 0 && d.pos[k - 1] > p) {
-        d.pos[k] = d.pos[k - 1];
-        --k;
-    }
-    d.pos[k] = (uint8_t)p;
-}
 void fix_bits(Deposit& d, uint64_t bits, bool one) {
-    for (uint64_t rest = bits; rest; rest &= rest - 1) add_pos(d, ctz(rest));
+    d.zeros |= bits;
+    d.n = popc(d.zeros);
     if (one) d.set |= bits;
 }
 
@@ -254,6 +247,8 @@ int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t gro
     const bool nt = cls == GClass::Diag || (partner & (line_amps - 1)) == 0;
     if (unroll_setting() == 1)
         return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d) : launch_kernel<R, 1, false>(r, gate, cls, groups, d);
+    if (unroll_setting() == 4)
+        return nt ? launch_kernel<R, 4, true>(r, gate, cls, groups, d) : launch_kernel<R, 4, false>(r, gate, cls, groups, d);
     return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d) : launch_kernel<R, 2, false>(r, gate, cls, groups, d);
 }
 
