@@ -378,7 +378,7 @@ typedef struct {
     uint32_t tune_caps[3];    /* the tile caps the autotuner tried: 10, 11, 12 (FP64) or 11, 12, 13 (FP32) */
     uint32_t reserved;
     double tune_ms[3][3];     /* autotuner: ms per run of the whole batch with one tile cap
-                                 (tune_caps) and one dense threshold (0, 2, 3) for every size
+                                 (tune_caps) and one dense threshold (0, 3, 4) for every size
                                  (0: not timed) */
     double single_ms[32];     /* autotuner: ms per run with registers of this size in one tile
                                  each, the others as chosen (0: not timed) */

@@ -19,7 +19,7 @@ class BatchInfo(ctypes.Structure):
 
 
 TUNE_CAPS = (10, 11, 12)          # FP64; FP32 tries 11, 12, 13
-TUNE_DENSE = (0, 2, 3)
+TUNE_DENSE = (0, 3, 4)
 
 
 def _declare():

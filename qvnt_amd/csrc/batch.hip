@@ -34,7 +34,7 @@ namespace {
 constexpr unsigned kMinQubits = qvnt::fusion::kMinFused;
 constexpr unsigned kMaxQubits = 30;
 constexpr unsigned kTuneCaps[] = {10, 11, 12};  // FP64; FP32 tiles are one qubit larger (tune_cap)
-constexpr unsigned kTuneDense[] = {0, 2, 3};
+constexpr unsigned kTuneDense[] = {0, 3, 4};
 
 size_t a256(size_t x) { return (x + 255) & ~size_t(255); }
 

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 
 namespace qvnt {
@@ -252,6 +253,18 @@ struct Left {
     }
 };
 
+// Qubits 0..row_bits-1 belong to every pass, so each tile row is that many
+// consecutive amplitudes in HBM (3: 128-byte lines in FP64).  QVNT_ROW_BITS
+// (1..4) overrides it for tuning.
+unsigned row_bits() {
+    static const unsigned r = [] {
+        const char* s = std::getenv("QVNT_ROW_BITS");
+        const int v = s ? std::atoi(s) : 3;
+        return (unsigned)(v >= 1 && v <= 4 ? v : 3);
+    }();
+    return r;
+}
+
 // Greedy pass formation (see the file comment).  A second scan with the
 // pass's final qubit set T leaves for a later pass every diagonal that
 // touches no qubit of T (in this pass it would only scale whole tiles; in
@@ -259,7 +272,8 @@ struct Left {
 // matrix primitive, so every pass makes progress.
 std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsigned tq) {
     std::vector<PassPlan> passes;
-    const uint64_t forced = (ns >= 3) ? 0x7u : ((1ull << ns) - 1u);
+    const unsigned rb = ns < row_bits() ? ns : row_bits();
+    const uint64_t forced = (1ull << rb) - 1u;
     while (!rest.empty()) {
         PassPlan pp;
         pp.T = forced;

@@ -9,7 +9,8 @@
 //      into one 2x2 (u3 = rz.ry.rz becomes one M2);
 //   3. schedule passes greedily: a pass owns a set T of at most `tq` qubits
 //      (always including qubits 0..2, so every tile row is whole 128-byte
-//      lines); walking the queue in order, a primitive joins the pass if its
+//      lines in FP64; QVNT_ROW_BITS changes the count); walking the queue
+//      in order, a primitive joins the pass if its
 //      matrix qubits fit in T (diagonals and controls may sit anywhere) and
 //      none of its qubits is held by an earlier primitive left out -- it
 //      then commutes with everything it overtakes;
