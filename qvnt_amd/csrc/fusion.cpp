@@ -16,6 +16,7 @@ using cd = std::complex<double>;
 struct Prim {
     uint8_t type;          // OP_M2 / OP_M4 / OP_D
     int q0 = -1, q1 = -1;  // M2: q0; M4: q0 (matrix bit 0), q1 (matrix bit 1); D: table bits
+    long src = -1;         // index of the SingleOp it comes from (-1: several)
     uint64_t ctrl = 0;
     cd m[16];              // M2 2x2, M4 4x4 (row-major), D table[4]
     uint64_t targets() const {
@@ -204,13 +205,14 @@ void mat2_mul(const cd* a, const cd* b, cd* out) {  // out = a * b
 // Multiplies runs of uncontrolled one-qubit primitives per qubit.
 std::vector<Prim> merge_single(const std::vector<Prim>& in) {
     std::vector<Prim> out;
-    struct Pend { bool on = false; cd m[4]; };
+    struct Pend { bool on = false; cd m[4]; long src = -1; };
     std::vector<Pend> pend(64);
     auto flush = [&](int q) {
         if (!pend[q].on) return;
         const cd* m = pend[q].m;
         if (m[1] == 0.0 && m[2] == 0.0) out.push_back(diag1(q, m[0], m[3], 0));
         else out.push_back(m2(q, m[0], m[1], m[2], m[3], 0));
+        out.back().src = pend[q].src;
         pend[q].on = false;
     };
     for (const Prim& p : in) {
@@ -220,8 +222,14 @@ std::vector<Prim> merge_single(const std::vector<Prim>& in) {
             if (p.type == OP_M2) { for (int k = 0; k < 4; ++k) u[k] = p.m[k]; }
             else { u[0] = p.m[0]; u[1] = 0.0; u[2] = 0.0; u[3] = p.m[1]; }
             Pend& pq = pend[p.q0];
-            if (pq.on) mat2_mul(u, pq.m, pq.m);
-            else { for (int k = 0; k < 4; ++k) pq.m[k] = u[k]; pq.on = true; }
+            if (pq.on) {
+                mat2_mul(u, pq.m, pq.m);
+                if (pq.src != p.src) pq.src = -1;
+            } else {
+                for (int k = 0; k < 4; ++k) pq.m[k] = u[k];
+                pq.on = true;
+                pq.src = p.src;
+            }
             continue;
         }
         for (uint64_t b = p.bits(); b; b &= b - 1) flush((int)ctz(b));
@@ -1035,7 +1043,8 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
 
 }  // namespace
 
-void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min) {
+void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min,
+          bool single_plain) {
     const uint64_t lmask = (1ull << ns) - 1u;
     size_t i = 0;
     while (i < n) {
@@ -1044,12 +1053,37 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
         size_t j = i;
         for (; j < n; ++j) {
             if ((g[j].ctrl & ~lmask) || g[j].kind == QVNT_ID) continue;
+            const size_t before = prims.size();
             if (!lower(g[j], lmask, prims)) break;
+            for (size_t k = before; k < prims.size(); ++k) prims[k].src = (long)j;
         }
         if (!prims.empty()) {
+            const std::vector<PassPlan> passes = schedule_passes(merge_single(prims), ns, tq);
+            // primitives per source SingleOp, to find passes that hold one
+            // whole SingleOp and nothing else: one plain launch moves at
+            // most the same bytes as a pass (fewer with controls) and skips
+            // the tile's LDS round trips
+            std::vector<uint32_t> per_src(n, 0);
+            for (const PassPlan& pp : passes)
+                for (const Prim& p : pp.prims)
+                    if (p.src >= 0) ++per_src[(size_t)p.src];
             Step st;
-            for (const PassPlan& pp : schedule_passes(merge_single(prims), ns, tq)) compile_pass(st.prog, pp, dense_min);
-            steps.push_back(std::move(st));
+            for (const PassPlan& pp : passes) {
+                const long src = pp.prims.empty() ? -1 : pp.prims[0].src;
+                bool single = single_plain && src >= 0 && per_src[(size_t)src] == pp.prims.size();
+                for (const Prim& p : pp.prims) single = single && p.src == src;
+                if (!single) {
+                    compile_pass(st.prog, pp, dense_min);
+                    continue;
+                }
+                if (!st.prog.passes.empty()) steps.push_back(std::move(st));
+                st = Step();
+                Step one;
+                one.plain = true;
+                one.gate = g[src];
+                steps.push_back(std::move(one));
+            }
+            if (!st.prog.passes.empty()) steps.push_back(std::move(st));
         }
         if (j < n) {
             Step st;

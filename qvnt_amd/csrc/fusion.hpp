@@ -48,6 +48,11 @@ constexpr unsigned max_tile(bool fp64) { return fp64 ? 12u : 13u; }
 constexpr unsigned kDefaultTile = 12;
 constexpr int kMinFused = 6;               // below 2^6 amplitudes the plain kernels run
 constexpr int kMinDenseTile = 10;          // dense phases need whole 64-lane waves: 2^(t-4) >= 64
+// From 2^20 amplitudes a pass is bandwidth-bound (16 MiB FP64), and a pass
+// that would hold a single SingleOp runs as that SingleOp's plain kernel
+// (same or fewer bytes, no LDS round trips); below, the single-register
+// fused form keeps every pass fused, like the batched form.
+constexpr unsigned kSinglePlainQubits = 20;
 
 // ---------------------------------------------------------- the program
 // M2: 2x2 on one register bit; M4: 4x4 on two; D: diagonal table;
@@ -147,8 +152,10 @@ struct Step {
 
 // Compiles a validated queue for a buffer of 2^ns amplitudes with tiles of
 // at most tq qubits.  Identity gates (Id, a control beyond the buffer) are
-// dropped.
-void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0);
+// dropped.  With single_plain, a pass that would hold one SingleOp and
+// nothing else becomes a plain step for that SingleOp.
+void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0,
+          bool single_plain = false);
 
 }  // namespace fusion
 }  // namespace qvnt

@@ -286,7 +286,7 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
     std::mt19937_64 g(seed);
     const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
-    plan(gates.data(), gates.size(), n, tq, steps, dense);
+    plan(gates.data(), gates.size(), n, tq, steps, dense, layered);  // layered: single-gate passes go plain
     orc_reg* ref = orc_reg_with_state(n, 0);
     const uint64_t len = orc_reg_len(ref);
     std::normal_distribution<double> nd;
@@ -294,9 +294,10 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
     for (auto& z : psi) z = C(nd(g), nd(g));
     std::memcpy(orc_reg_psi(ref), psi.data(), len * sizeof(C));
     orc_reg_apply(ref, gates.data(), gates.size());
-    unsigned passes = 0, dense_phases = 0, split = 0;
+    unsigned passes = 0, dense_phases = 0, split = 0, plain = 0;
     for (const Step& st : steps) {
         if (st.plain) {
+            ++plain;
             orc_reg* one = orc_reg_with_state(n, 0);  // a gate the fused kernel does not take
             std::memcpy(orc_reg_psi(one), psi.data(), len * sizeof(C));
             orc_reg_apply(one, &st.gate, 1);
@@ -314,8 +315,8 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
     const C* want = reinterpret_cast<const C*>(orc_reg_psi(ref));
     double err = 0.0;
     for (uint64_t i = 0; i < len; ++i) err = std::max(err, std::abs(psi[i] - want[i]));
-    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u dense phases, %u split/structured ops, max |d| %.2e\n", n, tq,
-                dense, gates.size(), passes, dense_phases, split, err);
+    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u plain steps, %u dense phases, %u split/structured ops, "
+                "max |d| %.2e\n", n, tq, dense, gates.size(), passes, plain, dense_phases, split, err);
     if (!(err <= 1e-12)) ++failures;
     orc_reg_free(ref);
 }
