@@ -92,6 +92,52 @@ def test_brickwork_and_measure(orc):
         assert r.measure_mask(0xFF0F, u=u).get() == want
 
 
+@pytest.mark.parametrize("n,prec", [(22, "f64"), (23, "f32")])
+def test_tfim_ring_fused(orc, n, prec):
+    """Trotter steps on a ring: the wrap-around rzz crosses tiles and
+    phases (factored two-qubit diagonals, controlled phases, diagonals
+    deferred to later passes); one call per step, then several steps in
+    one call, against the oracle."""
+    say("tfim ring n=%d %s" % (n, prec))
+    g = np.random.default_rng(900 + n)
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    step = circuits.tfim_step(n)
+    ref = orc.Reg(n, threads=8)
+    ref.set_state(psi)
+    for _ in range(5):
+        ref.apply(circuits.build(step, orc.op))
+    want = ref.state()
+    three = circuits.build(step, op)
+    for _ in range(2):
+        three *= circuits.build(step, op)
+    with QReg(n, prec=prec) as r:
+        r.set_state(psi)
+        for _ in range(2):
+            r.apply_fused(circuits.build(step, op))
+        r.apply_fused(three)
+        got = r.get_state().astype(np.complex128)
+    err = np.max(np.abs(got - want))
+    say("  max |d| = %.2e" % err)
+    assert err <= (TOL if prec == "f64" else 1e-5)
+
+
+def test_brickwork_layers_single_gate_passes(orc):
+    """One call per brickwork layer at 2^20 amplitudes: passes that would
+    hold only the layer's controlled x run as its plain kernel."""
+    n = 20
+    layers = [circuits.brickwork_layer(n, l) for l in range(6)]
+    ref = orc.Reg(n, threads=8)
+    for specs in layers:
+        ref.apply(circuits.build(specs, orc.op))
+    with QReg(n) as r:
+        passes = [r.apply_fused(circuits.build(specs, op)) for specs in layers]
+        got = r.get_state()
+    err = np.max(np.abs(got - ref.state()))
+    say("  passes per layer %s, max |d| = %.2e" % (passes, err))
+    assert err <= TOL
+
+
 def test_plain_20q_fused(orc):
     state, specs, mask = circuits.plain_20q()
     ref = orc.Reg(20, state, threads=8)
