@@ -36,6 +36,9 @@ static C cf(const Program& P, uint32_t off, int k) {
 // One register phase of one thread: the ops of fused.hip's run_phase.
 static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uint64_t tile) {
     CHECK(ph.op_begin <= ph.op_end && ph.op_end <= P.ops.size());
+    // the kernel runs a phase's G ops after all others: they must come last
+    for (uint32_t o = ph.op_begin; o + 1 < ph.op_end; ++o)
+        CHECK(P.ops[o].type != OP_G || P.ops[o + 1].type == OP_G);
     C scal = 1.0;
     for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
         const FOp& op = P.ops[o];
