@@ -139,6 +139,39 @@ __global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, u
   }
 }
 
+// Pair gates whose orbit mask lies in the lane bits (qubits 0..5, no
+// control among them): each lane loads one amplitude (a wave reads one
+// contiguous run), takes its partner's value from lane ^ a with a
+// cross-lane shuffle, and writes its own output -- the formula of the
+// reference for that amplitude (f_pair), so the result is the same to the
+// bit.  groups = amplitudes of the control subspace, a multiple of 64, so
+// every wave is either whole or idle and all of its lanes shuffle.
+template <int K, class R, int U>
+__global__ __launch_bounds__(kMaxBlock) void k_pair_lanes(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep,
+                                                          Par<R> g) {
+    const uint64_t B = blockDim.x, per = B * U;
+    const uint64_t nchunks = (groups + per - 1) / per;
+    const int lm = (int)g.a;
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint64_t base = chunk * per + threadIdx.x;
+        cplx<R> v[U];
+        uint64_t idx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t t = base + (uint64_t)u * B;
+            idx[u] = deposit(t, dep);
+            v[u] = cplx<R>{(R)0, (R)0};
+            if (t < groups) v[u] = ld<true>(psi + idx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const cplx<R> other{__shfl_xor(v[u].re, lm), __shfl_xor(v[u].im, lm)};
+            const uint64_t t = base + (uint64_t)u * B;
+            if (t < groups) st<true>(psi + idx[u], f_pair<K>(g, idx[u], v[u], other));
+        }
+    }
+}
+
 // QReg::with_state (quant.rs:129-150): zeros, psi[state] = 1.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_init(cplx<R>* __restrict__ psi, uint64_t len, uint64_t state) {
@@ -149,6 +182,14 @@ __global__ __launch_bounds__(kBlock) void k_init(cplx<R>* __restrict__ psi, uint
 // ----------------------------------------------------------- host side
 // Orbits per thread and threads per block of the gate kernels (tuning
 // knobs; the defaults are the measured best on a 2^30-amplitude state).
+// QVNT_PAIR_LANES=0 keeps low-qubit pair gates on k_pair (for comparison).
+bool lane_setting() {
+    static const bool on = [] {
+        const char* s = std::getenv("QVNT_PAIR_LANES");
+        return !(s && std::atoi(s) == 0);
+    }();
+    return on;
+}
 int unroll_setting() {
     static int u = [] {
         const char* s = std::getenv("QVNT_UNROLL");
@@ -241,6 +282,38 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
 // when the partner offsets are at least one line (8 FP64 or 16 FP32
 // amplitudes: the partners lie in other lines, read by other instructions).
 template <class R>
+int launch_lanes(qvnt_reg* r, const qvnt_gate& gate) {
+    Deposit d{};
+    fix_bits(d, gate.ctrl, true);
+    const uint64_t groups = r->len >> d.n;
+    const Par<R> p = make_par<R>(gate);
+    cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
+    const unsigned B = block_setting();
+    constexpr int U = 4;  // four amplitudes per lane: as many loads in flight as k_pair's two orbits
+    const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+    const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid)), block(B);
+    hipStream_t s = r->stream;
+#define QVNT_LANES(KIND)                                                                             \
+    case KIND:                                                                                       \
+        hipLaunchKernelGGL((k_pair_lanes<KIND, R, U>), grid, block, 0, s, psi, groups, d, p);       \
+        break;
+    switch (gate.kind) {
+        QVNT_LANES(QVNT_X)
+        QVNT_LANES(QVNT_Y)
+        QVNT_LANES(QVNT_RX)
+        QVNT_LANES(QVNT_RXX)
+        QVNT_LANES(QVNT_RY)
+        QVNT_LANES(QVNT_RYY)
+        QVNT_LANES(QVNT_H1)
+        QVNT_LANES(QVNT_U1)
+        default: return fail(QVNT_E_ARG, "not a lane-pair kind");
+    }
+#undef QVNT_LANES
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+template <class R>
 int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
     constexpr uint64_t line_amps = 128 / (2 * sizeof(R));
     const uint64_t partner = cls == GClass::Diag ? 0 : cls == GClass::Pair ? gate.a : (gate.a | gate.b);
@@ -325,6 +398,13 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
     const double bytes = 2.0 * (double)r->amp_bytes() * (double)(r->len >> popc(gate.ctrl));
     const int kc = cls == GClass::Diag ? QVNT_K_DIAG : cls == GClass::Pair ? QVNT_K_PAIR : QVNT_K_QUAD;
     Timed t(r, kc, bytes);
+    // partners within a wave's 64 consecutive amplitudes: lane exchange
+    const bool swap_family = k == QVNT_SWAP || k == QVNT_ISWAP || k == QVNT_SQRT_SWAP || k == QVNT_SQRT_ISWAP;
+    if (cls == GClass::Pair && !swap_family && (a_in & ~63ull) == 0 && (gate.ctrl & 63ull) == 0 &&
+        r->ns >= 6 + popc(gate.ctrl & lmask) && lane_setting()) {
+        if (r->prec == QVNT_F64) return launch_lanes<double>(r, gate);
+        return launch_lanes<float>(r, gate);
+    }
     if (r->prec == QVNT_F64) return launch_unrolled<double>(r, gate, cls, groups, d);
     return launch_unrolled<float>(r, gate, cls, groups, d);
 }
