@@ -190,6 +190,16 @@ bool lane_setting() {
     }();
     return on;
 }
+// pair and quad kernels: one orbit (2 or 4 amplitudes) per thread by
+// default, measured best at 30 qubits; QVNT_UNROLL_PAIR overrides
+int pair_unroll_setting() {
+    static int u = [] {
+        const char* s = std::getenv("QVNT_UNROLL_PAIR");
+        const int v = s ? std::atoi(s) : 1;
+        return (v == 1 || v == 2 || v == 4) ? v : 1;
+    }();
+    return u;
+}
 int unroll_setting() {
     static int u = [] {
         const char* s = std::getenv("QVNT_UNROLL");
@@ -318,9 +328,10 @@ int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t gro
     constexpr uint64_t line_amps = 128 / (2 * sizeof(R));
     const uint64_t partner = cls == GClass::Diag ? 0 : cls == GClass::Pair ? gate.a : (gate.a | gate.b);
     const bool nt = cls == GClass::Diag || (partner & (line_amps - 1)) == 0;
-    if (unroll_setting() == 1)
+    const int us = cls == GClass::Diag ? unroll_setting() : pair_unroll_setting();
+    if (us == 1)
         return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d) : launch_kernel<R, 1, false>(r, gate, cls, groups, d);
-    if (unroll_setting() == 4)
+    if (us == 4)
         return nt ? launch_kernel<R, 4, true>(r, gate, cls, groups, d) : launch_kernel<R, 4, false>(r, gate, cls, groups, d);
     return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d) : launch_kernel<R, 2, false>(r, gate, cls, groups, d);
 }
