@@ -60,6 +60,39 @@ def cpu_baseline(n, budget_s):
                       "out-of-place gather + per-apply copy), %.1f s" % (done, n, elapsed)}
 
 
+def config1_leg():
+    """The CPU-runnable case (20 qubits: qft on all qubits, h on all, 200
+    random gates; the state fits the CPU caches, so the oracle is not
+    page-fault bound here): the plain GPU form beside the oracle on all the
+    host threads used for cpu_baseline and on one thread."""
+    import oracle
+    from qvnt_amd import QReg, circuits, op
+    n = 20
+    state, specs, _ = circuits.plain_20q()
+    gops = circuits.build(specs, op)
+    count = len(gops)
+    reg = QReg.with_state(n, state)
+    reg.apply(gops)
+    reg.sync()
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 1.0:
+        reg.apply(gops)
+        reg.sync()
+        reps += 1
+    gpu = reps * count / (time.perf_counter() - t0)
+    reg.close()
+    cpu = {}
+    cops = circuits.build(specs, oracle.op)
+    for threads in sorted({min(os.cpu_count() or 1, 16), 1}, reverse=True):
+        ref = oracle.Reg(n, state, threads=threads)
+        t0 = time.perf_counter()
+        ref.apply(cops)
+        cpu[str(threads)] = round(count / (time.perf_counter() - t0), 1)
+        del ref
+    return {"workload": "plain-20q-fp64: qft(2^20-1), h on all qubits, 200 random gates (%d SingleOps)" % count,
+            "gpu_plain_gates_per_s": round(gpu, 1), "cpu_port_gates_per_s_by_threads": cpu}
+
+
 def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value, dense=0):
     """The same layers through qvnt_reg_apply_fused, one call per layer,
     then the whole timed circuit as one call (dense: the dense-phase
@@ -474,10 +507,14 @@ def main():
         fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, value, args.dense)
 
     cpu = None
+    c1 = None
     if not args.no_cpu:
         log("cpu baseline (oracle, host cores) ...")
         cpu = cpu_baseline(n, args.cpu_seconds)
         log("cpu: %.3f gates/s on %d threads" % (cpu["value"], cpu["cores"]))
+        c1 = config1_leg()
+        log("20-qubit case: GPU plain %.0f gates/s; oracle %s gates/s by threads"
+            % (c1["gpu_plain_gates_per_s"], c1["cpu_port_gates_per_s_by_threads"]))
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "gates/s", "n_gpus": 1,
@@ -491,6 +528,7 @@ def main():
         "all_kernels_gbs": round(all_bytes / all_ms / 1e6, 1) if all_ms else None,
         "fused": fused,
         "cpu_baseline": cpu,
+        "config1": c1,
     }
     print(json.dumps(line), flush=True)
 
