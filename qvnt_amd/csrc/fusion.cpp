@@ -278,40 +278,63 @@ unsigned row_bits() {
 // touches no qubit of T (in this pass it would only scale whole tiles; in
 // a later one its qubits may be registers), provided the pass keeps some
 // matrix primitive, so every pass makes progress.
-std::vector<PassPlan> schedule_passes(std::vector<Prim> rest, unsigned ns, unsigned tq) {
+std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns, unsigned tq) {
     std::vector<PassPlan> passes;
     const unsigned rb = ns < row_bits() ? ns : row_bits();
     const uint64_t forced = (1ull << rb) - 1u;
+    // Once the left-behind matrix primitives target every qubit the queue
+    // touches, every later primitive is blocked: a scan stops there.
+    uint64_t support = 0;
+    for (const Prim& p : all) support |= p.bits();
+    // the queue holds indices into `all`, in circuit order
+    std::vector<uint32_t> rest(all.size()), leave, take;
+    for (size_t k = 0; k < all.size(); ++k) rest[k] = (uint32_t)k;
+    // does a matrix primitive after queue position i target one of `bits`?
+    auto later_meets = [&](size_t i, uint64_t bits) {
+        for (size_t k = i + 1; k < rest.size(); ++k) {
+            const Prim& q = all[rest[k]];
+            if (q.type != OP_D && (q.targets() & bits)) return true;
+        }
+        return false;
+    };
     while (!rest.empty()) {
         PassPlan pp;
         pp.T = forced;
         Left left;
         bool matrix = false;
-        std::vector<Prim> leave;
-        for (const Prim& p : rest) {
-            if (left.blocks(p)) { left.add(p); leave.push_back(p); continue; }
+        leave.clear();
+        take.clear();
+        size_t i = 0;
+        for (; i < rest.size() && (left.mat_targets & support) != support; ++i) {
+            const Prim& p = all[rest[i]];
+            if (left.blocks(p)) { left.add(p); leave.push_back(rest[i]); continue; }
             const uint64_t nt = pp.T | p.targets();
-            if (popc(nt) <= tq) { pp.T = nt; pp.prims.push_back(p); matrix |= p.type != OP_D; }
-            else { left.add(p); leave.push_back(p); }
+            if (popc(nt) <= tq) { pp.T = nt; take.push_back(rest[i]); matrix |= p.type != OP_D; }
+            else { left.add(p); leave.push_back(rest[i]); }
         }
+        leave.insert(leave.end(), rest.begin() + (long)i, rest.end());
         // pad T with the lowest free qubits up to tq (larger, fewer tiles)
         for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
         if (matrix) {
-            // later matrix targets: a deferred diagonal must meet one of
-            // them, so some later pass holds its qubits anyway
-            std::vector<uint64_t> later(rest.size() + 1, 0);
-            for (size_t i = rest.size(); i-- > 0;)
-                later[i] = later[i + 1] | (i + 1 < rest.size() && rest[i + 1].type != OP_D ? rest[i + 1].targets() : 0);
-            pp.prims.clear();
+            // a deferred diagonal must meet a later matrix target, so some
+            // later pass holds its qubits anyway
+            take.clear();
             leave.clear();
             left = Left();
-            for (size_t i = 0; i < rest.size(); ++i) {
-                const Prim& p = rest[i];
-                const bool defer = p.type == OP_D && !(p.bits() & pp.T) && (p.bits() & later[i]);
-                if (left.blocks(p) || defer || (p.targets() & ~pp.T)) { left.add(p); leave.push_back(p); continue; }
-                pp.prims.push_back(p);
+            for (i = 0; i < rest.size() && (left.mat_targets & support) != support; ++i) {
+                const Prim& p = all[rest[i]];
+                if (left.blocks(p) || (p.targets() & ~pp.T) ||
+                    (p.type == OP_D && !(p.bits() & pp.T) && later_meets(i, p.bits()))) {
+                    left.add(p);
+                    leave.push_back(rest[i]);
+                    continue;
+                }
+                take.push_back(rest[i]);
             }
+            leave.insert(leave.end(), rest.begin() + (long)i, rest.end());
         }
+        pp.prims.reserve(take.size());
+        for (uint32_t k : take) pp.prims.push_back(all[k]);
         passes.push_back(std::move(pp));
         rest.swap(leave);
     }
@@ -392,13 +415,17 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
                     }
             }
         }
+        // T = G U; a row of G holds at most four nonzeros, summed in column order
         cd T[kRegs * kRegs];
-        for (int r = 0; r < kRegs; ++r)
-            for (int c = 0; c < kRegs; ++c) {
-                cd acc = 0.0;
-                for (int k = 0; k < kRegs; ++k) acc += G[r * kRegs + k] * U[k * kRegs + c];
-                T[r * kRegs + c] = acc;
+        for (int r = 0; r < kRegs; ++r) {
+            cd* t = T + r * kRegs;
+            for (int c = 0; c < kRegs; ++c) t[c] = 0.0;
+            for (int k = 0; k < kRegs; ++k) {
+                const cd gk = G[r * kRegs + k];
+                if (gk == 0.0) continue;
+                for (int c = 0; c < kRegs; ++c) t[c] += gk * U[k * kRegs + c];
             }
+        }
         for (int k = 0; k < kRegs * kRegs; ++k) U[k] = T[k];
     }
 }
@@ -413,14 +440,18 @@ uint32_t reg_targets(const FOp& op) {
 uint32_t diag_deps(const Program& prog, const FOp& op) {
     uint32_t dep = 0;
     const double* t = prog.coef.data() + op.coef;
-    for (int k = 0; k < 4; ++k)
-        for (int i = 0; i < kRegs; ++i)
-            for (int j = 0; j < kRegBits; ++j) {
-                const int i2 = i ^ (1 << j);
+    for (int j = 0; j < kRegBits; ++j) {
+        const int bit = 1 << j;
+        bool differs = false;  // some entry pair i, i | bit of some variant
+        for (int k = 0; k < 4 && !differs; ++k)
+            for (int i = 0; i < kRegs && !differs; ++i) {
+                if (i & bit) continue;
                 const double* a = t + 2 * (k * kRegs + i);
-                const double* b = t + 2 * (k * kRegs + i2);
-                if (a[0] != b[0] || a[1] != b[1]) dep |= 1u << j;
+                const double* b = a + 2 * bit;
+                differs = a[0] != b[0] || a[1] != b[1];
             }
+        if (differs) dep |= 1u << j;
+    }
     return dep;
 }
 
