@@ -376,57 +376,60 @@ FPhase make_phase(uint32_t Rl) {
 void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
     for (int r = 0; r < kRegs; ++r)
         for (int c = 0; c < kRegs; ++c) U[r * kRegs + c] = r == c ? 1.0 : 0.0;
+    // U <- G U for an op G that mixes the rows idx[0..m) (ascending) by the
+    // m x m block v and leaves every other row alone
+    auto mix = [&](const int* idx, int m, const cd* v) {
+        cd out[4 * kRegs];
+        for (int r = 0; r < m; ++r)
+            for (int c = 0; c < kRegs; ++c) {
+                cd acc = 0.0;
+                for (int k = 0; k < m; ++k)
+                    if (v[r * m + k] != 0.0) acc += v[r * m + k] * U[idx[k] * kRegs + c];
+                out[r * kRegs + c] = acc;
+            }
+        for (int r = 0; r < m; ++r)
+            for (int c = 0; c < kRegs; ++c) U[idx[r] * kRegs + c] = out[r * kRegs + c];
+    };
     for (uint32_t o = op_begin; o < op_end; ++o) {
         const FOp& op = prog.ops[o];
         auto cf = [&](int k) { return cd(prog.coef[op.coef + 2 * k], prog.coef[op.coef + 2 * k + 1]); };
-        cd G[kRegs * kRegs];
-        for (int r = 0; r < kRegs; ++r)
-            for (int c = 0; c < kRegs; ++c) G[r * kRegs + c] = r == c ? 1.0 : 0.0;
         if (op.type == OP_D) {
-            for (int i = 0; i < kRegs; ++i) G[i * kRegs + i] = cf(i);
+            for (int i = 0; i < kRegs; ++i) {
+                const int idx[1] = {i};
+                const cd v[1] = {cf(i)};
+                mix(idx, 1, v);
+            }
         } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2) {
             const int J = (int)op.sel;
             const cd one(1.0, 0.0), zero(0.0, 0.0);
             const cd u00 = op.type == OP_M2 ? cf(0) : zero, u11 = op.type == OP_M2 ? cf(3) : zero;
             const cd u01 = op.type == OP_M2 ? cf(1) : op.type == OP_A2 ? cf(0) : one;
             const cd u10 = op.type == OP_M2 ? cf(2) : op.type == OP_A2 ? cf(1) : one;
+            const cd v[4] = {u00, u01, u10, u11};
             for (int i = 0; i < kRegs; ++i) {
                 if (i & (1 << J)) continue;
                 if (op.pc != 0xFFFFu && !((op.pc >> i) & 1u)) continue;
-                const int i1 = i | (1 << J);
-                G[i * kRegs + i] = u00;
-                G[i * kRegs + i1] = u01;
-                G[i1 * kRegs + i] = u10;
-                G[i1 * kRegs + i1] = u11;
+                const int idx[2] = {i, i | (1 << J)};
+                mix(idx, 2, v);
             }
         } else {
-            const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;
+            const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;  // j0 < j1
+            cd v[16];
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) {
+                    cd x = 0.0;
+                    if (op.type == OP_M4) x = cf(4 * r + c);
+                    else if (c == r) x = cf(2 * r);
+                    else if (c == (r ^ 3)) x = cf(2 * r + 1);
+                    v[4 * r + c] = x;
+                }
             for (int i = 0; i < kRegs; ++i) {
                 if (i & ((1 << j0) | (1 << j1))) continue;
                 if (!((op.pc >> i) & 1u)) continue;
                 const int idx[4] = {i, i | (1 << j0), i | (1 << j1), i | (1 << j0) | (1 << j1)};
-                for (int r = 0; r < 4; ++r)
-                    for (int c = 0; c < 4; ++c) {
-                        cd v = 0.0;
-                        if (op.type == OP_M4) v = cf(4 * r + c);
-                        else if (c == r) v = cf(2 * r);
-                        else if (c == (r ^ 3)) v = cf(2 * r + 1);
-                        G[idx[r] * kRegs + idx[c]] = v;
-                    }
+                mix(idx, 4, v);
             }
         }
-        // T = G U; a row of G holds at most four nonzeros, summed in column order
-        cd T[kRegs * kRegs];
-        for (int r = 0; r < kRegs; ++r) {
-            cd* t = T + r * kRegs;
-            for (int c = 0; c < kRegs; ++c) t[c] = 0.0;
-            for (int k = 0; k < kRegs; ++k) {
-                const cd gk = G[r * kRegs + k];
-                if (gk == 0.0) continue;
-                for (int c = 0; c < kRegs; ++c) t[c] += gk * U[k * kRegs + c];
-            }
-        }
-        for (int k = 0; k < kRegs * kRegs; ++k) U[k] = T[k];
     }
 }
 
@@ -526,14 +529,26 @@ void ratio_table(Program& prog, FOp& op, const FPhase& ph) {
 // targets one of its bits (its window of slots; slot s lies before matrix
 // op s).  In order of window end, each diagonal joins the first group
 // placed inside its window whose table it merges with, or opens a group at
-// its window's end; every group is one table.
-void merge_diagonals(Program& prog, uint32_t begin, bool separate_ext = false) {
+// its window's end; every group is one table.  `deps` caches diag_deps by
+// coefficient offset for tables that hold the same values whenever this is
+// called with that cache.
+using DepCache = std::vector<std::pair<uint32_t, uint32_t>>;
+void merge_diagonals(Program& prog, uint32_t begin, bool separate_ext, DepCache& deps) {
+    auto dep_of = [&](const FOp& op) {
+        for (const auto& e : deps)
+            if (e.first == op.coef) return e.second;
+        const uint32_t d = diag_deps(prog, op);
+        deps.push_back({op.coef, d});
+        return d;
+    };
     std::vector<FOp> mats;
     struct Diag { FOp op; uint32_t dep, lo, hi, at; };
     std::vector<Diag> diags;
+    mats.reserve(prog.ops.size() - begin);
+    diags.reserve(prog.ops.size() - begin);
     for (size_t o = begin; o < prog.ops.size(); ++o) {
         const FOp& op = prog.ops[o];
-        if (op.type == OP_D) diags.push_back({op, diag_deps(prog, op), 0, 0, (uint32_t)mats.size()});
+        if (op.type == OP_D) diags.push_back({op, dep_of(op), 0, 0, (uint32_t)mats.size()});
         else mats.push_back(op);
     }
     const uint32_t m = (uint32_t)mats.size();
@@ -768,11 +783,20 @@ void phase_on_bit(Program& prog, FOp& op) {
 // and without split 2x2s, whichever is cheaper; then the ratio tables.
 void finish_phase(Program& prog, uint32_t begin, size_t coef_begin, const FPhase& ph) {
     std::vector<FOp> scalars;
+    // Every variant below starts from the same tables and appends the same
+    // new ones at the same offsets before merging (tables are rewritten
+    // only while merging), so the merges share one cache of their deps.
+    DepCache deps;
     {
         std::vector<FOp> rest;
         for (size_t o = begin; o < prog.ops.size(); ++o) {
             const FOp& op = prog.ops[o];
-            if (op.type == OP_D && !op.la && !op.lb && !op.lctrl && diag_deps(prog, op) == 0) {
+            uint32_t dep = 1;
+            if (op.type == OP_D) {
+                dep = diag_deps(prog, op);
+                deps.push_back({op.coef, dep});
+            }
+            if (op.type == OP_D && !op.la && !op.lb && !op.lctrl && dep == 0) {
                 FOp g = op;
                 g.type = OP_G;
                 scalars.push_back(g);
@@ -792,13 +816,17 @@ void finish_phase(Program& prog, uint32_t begin, size_t coef_begin, const FPhase
     std::vector<FOp> best_ops;
     std::vector<double> best_coef;
     uint32_t best = 0xFFFFFFFFu;
-    for (int variant = 0; variant < 3; ++variant) {
+    // with every diagonal on register bits only (as the ones splitting adds
+    // are), keeping the others apart changes nothing: variant 2 is variant 1
+    bool all_plain = true;
+    for (const FOp& op : ops0) all_plain = all_plain && (op.type != OP_D || plain_diag(op));
+    for (int variant = 0; variant < (all_plain ? 2 : 3); ++variant) {
         prog.ops.resize(begin);
         prog.ops.insert(prog.ops.end(), ops0.begin(), ops0.end());
         prog.coef.resize(coef_begin);
         prog.coef.insert(prog.coef.end(), coef0.begin(), coef0.end());
         if (variant > 0) split_pairs(prog, begin);
-        merge_diagonals(prog, begin, variant == 2);
+        merge_diagonals(prog, begin, variant == 2, deps);
         if (variant > 0) fold_structured(prog, begin);
         const uint32_t cost = phase_cost(prog, begin);
         if (cost < best) {
@@ -912,25 +940,36 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     const uint32_t top_regs = ((1u << kRegBits) - 1u) << (fp.t - kRegBits);  // local bits t-4..t-1
 
     // phases: greedy register sets of four local qubits
-    std::vector<Prim> rest = pp.prims;
+    // (the queue holds indices into pp.prims; a scan stops once the
+    // left-behind matrix primitives target every qubit the pass touches)
+    const std::vector<Prim>& all = pp.prims;
+    uint64_t support = 0;
+    for (const Prim& p : all) support |= p.bits();
+    std::vector<uint32_t> rest(all.size()), leave;
+    for (size_t k = 0; k < all.size(); ++k) rest[k] = (uint32_t)k;
     std::vector<PhasePlan> plans;
     while (!rest.empty()) {
         uint32_t Rl = 0;  // local register bits
         Left left;
-        std::vector<Prim> take, leave;
-        for (const Prim& p : rest) {
-            if (left.blocks(p)) { left.add(p); leave.push_back(p); continue; }
-            if (p.type == OP_D) { take.push_back(p); continue; }
+        PhasePlan plan;
+        leave.clear();
+        size_t i = 0;
+        for (; i < rest.size() && (left.mat_targets & support) != support; ++i) {
+            const Prim& p = all[rest[i]];
+            if (left.blocks(p)) { left.add(p); leave.push_back(rest[i]); continue; }
+            if (p.type == OP_D) { plan.prims.push_back(p); continue; }
             uint32_t need = 0;
             for (uint64_t m = p.targets(); m; m &= m - 1) need |= 1u << local_of(fp.tpos, fp.t, (int)ctz(m));
-            if (popc(Rl | need) <= kRegBits) { Rl |= need; take.push_back(p); }
-            else { left.add(p); leave.push_back(p); }
+            if (popc(Rl | need) <= kRegBits) { Rl |= need; plan.prims.push_back(p); }
+            else { left.add(p); leave.push_back(rest[i]); }
         }
+        leave.insert(leave.end(), rest.begin() + (long)i, rest.end());
         // complete the register set, preferring the top local bits (direct HBM access)
         for (int k = (int)fp.t - 1; k >= 0 && popc(Rl) < kRegBits; --k)
             if ((top_regs >> k) & 1) Rl |= 1u << k;
         for (int k = 0; k < (int)fp.t && popc(Rl) < kRegBits; ++k) Rl |= 1u << k;
-        plans.push_back({Rl, std::move(take)});
+        plan.Rl = Rl;
+        plans.push_back(std::move(plan));
         rest.swap(leave);
     }
     relocate_diagonals(plans, fp);
