@@ -620,7 +620,7 @@ int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, 
     if (tq < (unsigned)kMinFused) tq = kMinFused;
     if (tq > r->ns) tq = r->ns;
     std::vector<Step> steps;
-    plan(g, n, r->ns, tq, steps, r->dense_min, r->ns >= kSinglePlainQubits);
+    plan(g, n, r->ns, tq, steps, r->dense_min, r->ns >= kSinglePlainQubits, true);
     uint32_t count = 0;
     for (const Step& st : steps) {
         if (st.plain) {

@@ -3,9 +3,11 @@
 #include "fusion.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace qvnt {
 namespace fusion {
@@ -374,21 +376,37 @@ FPhase make_phase(uint32_t Rl) {
 // a phase whose ops touch register bits only, in the kernel's semantics
 // (fused.hip: apply_m2 / apply_m4 / apply_d with table variant 0).
 void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
+    // real and imaginary parts apart; each product and sum is the one
+    // std::complex forms (re = ac - bd, im = ad + bc), without its NaN checks
+    double ur[kRegs][kRegs], ui[kRegs][kRegs];
     for (int r = 0; r < kRegs; ++r)
-        for (int c = 0; c < kRegs; ++c) U[r * kRegs + c] = r == c ? 1.0 : 0.0;
+        for (int c = 0; c < kRegs; ++c) {
+            ur[r][c] = r == c ? 1.0 : 0.0;
+            ui[r][c] = 0.0;
+        }
     // U <- G U for an op G that mixes the rows idx[0..m) (ascending) by the
     // m x m block v and leaves every other row alone
     auto mix = [&](const int* idx, int m, const cd* v) {
-        cd out[4 * kRegs];
+        double outr[4][kRegs], outi[4][kRegs];
+        for (int r = 0; r < m; ++r) {
+            for (int c = 0; c < kRegs; ++c) outr[r][c] = outi[r][c] = 0.0;
+            for (int k = 0; k < m; ++k) {
+                const cd g = v[r * m + k];
+                if (g == 0.0) continue;
+                const double gr = g.real(), gi = g.imag();
+                const double* xr = ur[idx[k]];
+                const double* xi = ui[idx[k]];
+                for (int c = 0; c < kRegs; ++c) {
+                    outr[r][c] += gr * xr[c] - gi * xi[c];
+                    outi[r][c] += gr * xi[c] + gi * xr[c];
+                }
+            }
+        }
         for (int r = 0; r < m; ++r)
             for (int c = 0; c < kRegs; ++c) {
-                cd acc = 0.0;
-                for (int k = 0; k < m; ++k)
-                    if (v[r * m + k] != 0.0) acc += v[r * m + k] * U[idx[k] * kRegs + c];
-                out[r * kRegs + c] = acc;
+                ur[idx[r]][c] = outr[r][c];
+                ui[idx[r]][c] = outi[r][c];
             }
-        for (int r = 0; r < m; ++r)
-            for (int c = 0; c < kRegs; ++c) U[idx[r] * kRegs + c] = out[r * kRegs + c];
     };
     for (uint32_t o = op_begin; o < op_end; ++o) {
         const FOp& op = prog.ops[o];
@@ -431,6 +449,8 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
             }
         }
     }
+    for (int r = 0; r < kRegs; ++r)
+        for (int c = 0; c < kRegs; ++c) U[r * kRegs + c] = cd(ur[r][c], ui[r][c]);
 }
 
 uint32_t reg_targets(const FOp& op) {
@@ -1111,10 +1131,57 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     prog.passes.push_back(fp);
 }
 
+// Appends `src` (offsets from 0) to `dst`, moving its phase, op and
+// coefficient offsets past what `dst` holds.
+void append_program(Program& dst, const Program& src) {
+    const uint32_t pb = (uint32_t)dst.phases.size(), ob = (uint32_t)dst.ops.size(), cb = (uint32_t)dst.coef.size();
+    for (FPass fp : src.passes) {
+        fp.phase_begin += pb;
+        fp.phase_end += pb;
+        dst.passes.push_back(fp);
+    }
+    for (FPhase ph : src.phases) {
+        ph.op_begin += ob;
+        ph.op_end += ob;
+        if (ph.dense) ph.ucoef += cb;
+        dst.phases.push_back(ph);
+    }
+    for (FOp op : src.ops) {
+        if (coef_complex(op)) op.coef += cb;
+        dst.ops.push_back(op);
+    }
+    dst.coef.insert(dst.coef.end(), src.coef.begin(), src.coef.end());
+}
+
+// With parallel, passes are compiled on up to kCompileThreads threads (the
+// caller's among them) when the queue holds at least kParallelPrims
+// primitives to compile.
+constexpr unsigned kCompileThreads = 8;
+constexpr size_t kParallelPrims = 64;
+
+template <class F>
+void parallel_for(size_t n, bool parallel, const F& fn) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    size_t nt = parallel ? std::min<size_t>(n, std::min(hw ? hw : 1u, kCompileThreads)) : 1;
+    if (nt <= 1) {
+        for (size_t k = 0; k < n; ++k) fn(k);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto run = [&] {
+        for (size_t k; (k = next.fetch_add(1, std::memory_order_relaxed)) < n;) fn(k);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (size_t t = 1; t < nt; ++t) pool.emplace_back(run);
+    run();
+    for (std::thread& t : pool) t.join();
+}
+
 }  // namespace
 
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min,
-          bool single_plain) {
+          bool single_plain, bool parallel) {
     const uint64_t lmask = (1ull << ns) - 1u;
     size_t i = 0;
     while (i < n) {
@@ -1137,20 +1204,33 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
             for (const PassPlan& pp : passes)
                 for (const Prim& p : pp.prims)
                     if (p.src >= 0) ++per_src[(size_t)p.src];
-            Step st;
-            for (const PassPlan& pp : passes) {
+            std::vector<long> single_src(passes.size(), -1);
+            size_t work = 0;
+            for (size_t k = 0; k < passes.size(); ++k) {
+                const PassPlan& pp = passes[k];
                 const long src = pp.prims.empty() ? -1 : pp.prims[0].src;
                 bool single = single_plain && src >= 0 && per_src[(size_t)src] == pp.prims.size();
                 for (const Prim& p : pp.prims) single = single && p.src == src;
-                if (!single) {
-                    compile_pass(st.prog, pp, dense_min);
+                if (single) single_src[k] = src;
+                else work += pp.prims.size();
+            }
+            // passes compile independently, each into a program of its own
+            // (offsets from 0), concatenated in order afterwards
+            std::vector<Program> progs(passes.size());
+            parallel_for(passes.size(), parallel && work >= kParallelPrims, [&](size_t k) {
+                if (single_src[k] < 0) compile_pass(progs[k], passes[k], dense_min);
+            });
+            Step st;
+            for (size_t k = 0; k < passes.size(); ++k) {
+                if (single_src[k] < 0) {
+                    append_program(st.prog, progs[k]);
                     continue;
                 }
                 if (!st.prog.passes.empty()) steps.push_back(std::move(st));
                 st = Step();
                 Step one;
                 one.plain = true;
-                one.gate = g[src];
+                one.gate = g[single_src[k]];
                 steps.push_back(std::move(one));
             }
             if (!st.prog.passes.empty()) steps.push_back(std::move(st));

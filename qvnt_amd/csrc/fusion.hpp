@@ -153,9 +153,11 @@ struct Step {
 // Compiles a validated queue for a buffer of 2^ns amplitudes with tiles of
 // at most tq qubits.  Identity gates (Id, a control beyond the buffer) are
 // dropped.  With single_plain, a pass that would hold one SingleOp and
-// nothing else becomes a plain step for that SingleOp.
+// nothing else becomes a plain step for that SingleOp.  With parallel, the
+// passes of a long queue compile on several host threads (the output is
+// the same); callers that already compile many queues at once leave it off.
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0,
-          bool single_plain = false);
+          bool single_plain = false, bool parallel = false);
 
 }  // namespace fusion
 }  // namespace qvnt

@@ -285,11 +285,49 @@ static std::vector<qvnt_gate> layered_circuit(std::mt19937_64& g, unsigned n, in
     return out;
 }
 
+// Programs equal field by field (coefficients bit for bit).
+static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t k = 0; k < a.size(); ++k) {
+        const Step &x = a[k], &y = b[k];
+        if (x.plain != y.plain) return false;
+        if (x.plain) {
+            if (std::memcmp(&x.gate, &y.gate, sizeof x.gate) != 0) return false;
+            continue;
+        }
+        const Program &p = x.prog, &q = y.prog;
+        if (p.passes.size() != q.passes.size() || p.phases.size() != q.phases.size() ||
+            p.ops.size() != q.ops.size() || p.coef.size() != q.coef.size())
+            return false;
+        for (size_t i = 0; i < p.passes.size(); ++i) {
+            const FPass &u = p.passes[i], &v = q.passes[i];
+            if (u.t != v.t || u.nthreads != v.nthreads || u.phase_begin != v.phase_begin ||
+                u.phase_end != v.phase_end || u.direct_first != v.direct_first || u.direct_last != v.direct_last ||
+                std::memcmp(u.tpos, v.tpos, sizeof u.tpos) != 0 || std::memcmp(u.hoff, v.hoff, sizeof u.hoff) != 0)
+                return false;
+        }
+        if (std::memcmp(p.phases.data(), q.phases.data(), p.phases.size() * sizeof(FPhase)) != 0 ||
+            std::memcmp(p.ops.data(), q.ops.data(), p.ops.size() * sizeof(FOp)) != 0 ||
+            std::memcmp(p.coef.data(), q.coef.data(), p.coef.size() * sizeof(double)) != 0)
+            return false;
+    }
+    return true;
+}
+
 static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false) {
     std::mt19937_64 g(seed);
     const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
     plan(gates.data(), gates.size(), n, tq, steps, dense, layered);  // layered: single-gate passes go plain
+    {
+        // passes compiled on several threads and concatenated: the same program
+        std::vector<Step> par;
+        plan(gates.data(), gates.size(), n, tq, par, dense, layered, true);
+        if (!same_steps(steps, par)) {
+            std::printf("n=%u tq=%u dense=%u: the parallel compile differs\n", n, tq, dense);
+            ++failures;
+        }
+    }
     orc_reg* ref = orc_reg_with_state(n, 0);
     const uint64_t len = orc_reg_len(ref);
     std::normal_distribution<double> nd;
