@@ -63,8 +63,9 @@ def cpu_baseline(n, budget_s):
 def config1_leg():
     """The CPU-runnable case (20 qubits: qft on all qubits, h on all, 200
     random gates; the state fits the CPU caches, so the oracle is not
-    page-fault bound here): the plain GPU form beside the oracle on all the
-    host threads used for cpu_baseline and on one thread."""
+    page-fault bound here): the plain GPU form and the fused form (compiled
+    again on every call) beside the oracle on all the host threads used for
+    cpu_baseline and on one thread."""
     import oracle
     from qvnt_amd import QReg, circuits, op
     n = 20
@@ -81,6 +82,24 @@ def config1_leg():
         reps += 1
     gpu = reps * count / (time.perf_counter() - t0)
     reg.close()
+    # the fused form of the same queue (each call compiles it again), and
+    # its agreement with the plain form from the same start state
+    import numpy as np
+    with QReg.with_state(n, state) as a, QReg.with_state(n, state) as b:
+        a.apply(gops)
+        b.apply_fused(gops)
+        fused_diff = float(np.max(np.abs(a.get_state() - b.get_state())))
+    if not fused_diff <= 1e-12:
+        raise RuntimeError("20-qubit case: fused and plain forms differ by %.3g" % fused_diff)
+    with QReg.with_state(n, state) as reg:
+        reg.apply_fused(gops)
+        reg.sync()
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 1.0:
+            reg.apply_fused(gops)
+            reg.sync()
+            reps += 1
+        fused = reps * count / (time.perf_counter() - t0)
     cpu = {}
     cops = circuits.build(specs, oracle.op)
     for threads in sorted({min(os.cpu_count() or 1, 16), 1}, reverse=True):
@@ -90,7 +109,8 @@ def config1_leg():
         cpu[str(threads)] = round(count / (time.perf_counter() - t0), 1)
         del ref
     return {"workload": "plain-20q-fp64: qft(2^20-1), h on all qubits, 200 random gates (%d SingleOps)" % count,
-            "gpu_plain_gates_per_s": round(gpu, 1), "cpu_port_gates_per_s_by_threads": cpu}
+            "gpu_plain_gates_per_s": round(gpu, 1), "gpu_fused_gates_per_s": round(fused, 1),
+            "fused_vs_plain_max_abs_diff": fused_diff, "cpu_port_gates_per_s_by_threads": cpu}
 
 
 def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value, dense=0):
@@ -408,6 +428,62 @@ def load_traffic(kernel):
     return data.get(kernel, {}).get("bytes_per_launch")
 
 
+def tiles(args):
+    """Tile sizes of the fused form at --qubits (default 30): four workloads
+    (the brickwork, 40 layers, one call per layer and one call in all; the
+    TFIM ring, 50 steps, one call per step and one call in all) run in tiles
+    of 10, 11 and 12 qubits and with the automatic choice, each timed
+    (wall clock around the calls, compile included).  Per tile size, a least
+    squares fit of time = passes x a + cost x b over the four workloads
+    (passes and the compiler's summed op cost from qvnt_fused_plan_stats):
+    the model behind the automatic choice."""
+    import numpy as np
+    from qvnt_amd import QReg, circuits, fused_plan_stats, op
+    n, prec = args.qubits, args.prec
+
+    def merged(lst):
+        m = lst[0].clone()
+        for o in lst[1:]:
+            m *= o
+        return m
+
+    bw = [circuits.build(circuits.brickwork_layer(n, l), op) for l in range(40)]
+    tf = circuits.build(circuits.tfim_step(n), op)
+    work = {"brickwork_per_layer": bw, "brickwork_whole": [merged(bw)], "tfim_per_step": [tf] * 50,
+            "tfim_whole": [merged([tf] * 50)]}
+    times, stats = {}, {}
+    for name, calls in work.items():
+        for cap in (10, 11, 12, 0):
+            reg = QReg(n, prec=prec)
+            reg.apply_fused(calls[0], max_qubits=cap)  # first-call allocations
+            reg.sync()
+            t0 = time.perf_counter()
+            passes = 0
+            for c in calls:
+                passes += reg.apply_fused(c, max_qubits=cap)
+            reg.sync()
+            dt = time.perf_counter() - t0
+            reg.close()
+            st = [fused_plan_stats(c, n, prec, max_qubits=cap) for c in calls]
+            stats[(name, cap)] = (sum(x["passes"] for x in st), sum(x["cost"] for x in st))
+            assert stats[(name, cap)][0] == passes, (name, cap)
+            times[(name, cap)] = dt
+            log("%s %s tile %s: %d passes, %.4f s" % (prec, name, cap or "auto", passes, dt))
+    fit = {}
+    for cap in (10, 11, 12):
+        A = np.array([stats[(w, cap)] for w in work], dtype=float)
+        y = np.array([times[(w, cap)] for w in work])
+        coef = np.linalg.lstsq(A, y, rcond=None)[0]
+        fit[cap] = {"pass_s": float(coef[0]), "cost_s": float(coef[1]),
+                    "max_rel_err": float(np.max(np.abs(A @ coef - y) / y))}
+        log("fit tile %d: %.4e s/pass + %.4e s/cost, max rel. error %.3f" % (cap, coef[0], coef[1],
+                                                                             fit[cap]["max_rel_err"]))
+    best = {w: min((10, 11, 12), key=lambda c: times[(w, c)]) for w in work}
+    print(json.dumps({"workload": "tiles", "qubits": n, "dtype": prec,
+                      "seconds": {"%s/%s" % (w, c or "auto"): round(t, 4) for (w, c), t in times.items()},
+                      "fastest_fixed_tile": best, "fit": fit}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,7 +495,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch"])
+    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
     ap.add_argument("--dense", type=int, default=None,
@@ -442,7 +518,7 @@ def main():
         args.dense = 3 if args.workload == "serve" else 0
     if args.qubits is None:
         args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
-                       "batch": 20}[args.workload]
+                       "batch": 20, "tiles": 30}[args.workload]
 
     import qvnt_amd
     from qvnt_amd import QReg, circuits, op
@@ -455,6 +531,8 @@ def main():
         return serve(args)
     if args.workload == "batch":
         return batch(args)
+    if args.workload == "tiles":
+        return tiles(args)
 
     n = args.qubits
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]
@@ -513,8 +591,9 @@ def main():
         cpu = cpu_baseline(n, args.cpu_seconds)
         log("cpu: %.3f gates/s on %d threads" % (cpu["value"], cpu["cores"]))
         c1 = config1_leg()
-        log("20-qubit case: GPU plain %.0f gates/s; oracle %s gates/s by threads"
-            % (c1["gpu_plain_gates_per_s"], c1["cpu_port_gates_per_s_by_threads"]))
+        log("20-qubit case: GPU plain %.0f gates/s, fused %.0f gates/s (max |d| vs plain %.1e); oracle %s gates/s "
+            "by threads" % (c1["gpu_plain_gates_per_s"], c1["gpu_fused_gates_per_s"], c1["fused_vs_plain_max_abs_diff"],
+                            c1["cpu_port_gates_per_s_by_threads"]))
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "gates/s", "n_gpus": 1,

@@ -184,11 +184,23 @@ int qvnt_reg_apply_gates(qvnt_reg* reg, const qvnt_gate* gates, size_t count);
  * runs one gather pass per SingleOp).  The queue is scheduled into passes;
  * each pass reads and writes the state once and applies every gate it
  * absorbed while tiles of 2^t amplitudes (t = max_qubits, at most 12 for
- * FP64 and 13 for FP32; 0 = 12) sit in LDS and registers.  Gates are reordered only past gates on
+ * FP64 and 13 for FP32) sit in LDS and registers.  max_qubits = 0: t = 12,
+ * except from 28 qubits without dense phases, where the queue is compiled
+ * for t = 10, 11 and 12 and the program with the lowest estimated run time
+ * is kept (a model of passes and arithmetic fitted to measured runs; long
+ * compute-heavy queues run faster in smaller tiles).  Gates are reordered only past gates on
  * disjoint qubits.  Same result as qvnt_reg_apply to rounding (FMA and
  * merged 2x2 products: about 1e-15 per gate).  *passes (may be NULL) gets
  * the number of passes over the state. */
 int qvnt_reg_apply_fused(qvnt_reg* reg, const qvnt_op* op, uint32_t max_qubits, uint32_t* passes);
+/* What qvnt_reg_apply_fused would compile for a register of q_num qubits of
+ * precision prec with this max_qubits and dense-phase threshold (no
+ * reference counterpart; host only, no device needed): out[0] = passes over
+ * the state (plain launches counted), out[1] = tile qubits used (0: plain
+ * kernels only), out[2] = the compiler's summed cost estimate of the ops
+ * (vector instructions per thread), out[3] = dense phases. */
+int qvnt_fused_plan_stats(const qvnt_op* op, uint32_t q_num, qvnt_prec prec, uint32_t max_qubits, uint32_t dense_min,
+                          double* out);
 
 /* QReg::get_absolute (quant.rs:458-466): sum of |psi_i|^2 over the buffer. */
 int qvnt_reg_get_absolute(qvnt_reg* reg, double* out);

@@ -87,6 +87,7 @@ def _declare(L):
         "qvnt_reg_apply": ([vp, vp], i),
         "qvnt_reg_apply_gates": ([vp, gp, sz], i),
         "qvnt_reg_apply_fused": ([vp, vp, u32, ctypes.POINTER(u32)], i),
+        "qvnt_fused_plan_stats": ([vp, u32, i, u32, u32, dp], i),
         "qvnt_reg_get_absolute": ([vp, dp], i),
         "qvnt_reg_get_probabilities": ([vp, dp, u64], i),
         "qvnt_reg_measure_mask": ([vp, u64, d, u32, u64p], i),

@@ -604,7 +604,45 @@ int run_program(qvnt_reg* r, const Program& prog) {
     return launch_program(r, prog, r->prog);
 }
 
+// The steps of a fused apply on 2^ns (>= 2^kMinFused) amplitudes; returns
+// the tile size.  max_qubits = 0 and no dense phases from kAutoTileQubits:
+// the tile the run-time model picks; otherwise max_qubits (0: the default)
+// clamped to what the precision's LDS tile and the buffer allow.
+unsigned compile_fused(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,
+                       std::vector<Step>& steps) {
+    const bool single_plain = ns >= kSinglePlainQubits;
+    if (!max_qubits && dense_min == 0 && ns >= kAutoTileQubits)
+        return plan_auto(g, n, ns, fp64, steps, single_plain, true);
+    const unsigned cap = max_tile(fp64);
+    unsigned tq = max_qubits ? max_qubits : kDefaultTile;
+    if (tq > cap) tq = cap;
+    if (tq < (unsigned)kMinFused) tq = kMinFused;
+    if (tq > ns) tq = ns;
+    plan(g, n, ns, tq, steps, dense_min, single_plain, true);
+    return tq;
+}
+
 }  // namespace
+
+void fused_plan_stats(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,
+                      double* out) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    if (ns < (unsigned)kMinFused) {  // the plain kernels, one launch per gate
+        out[0] = (double)n;
+        return;
+    }
+    std::vector<Step> steps;
+    out[1] = compile_fused(g, n, ns, fp64, max_qubits, dense_min, steps);
+    for (const Step& st : steps) {
+        if (st.plain) {
+            out[0] += 1.0;
+            continue;
+        }
+        out[0] += (double)st.prog.passes.size();
+        out[2] += (double)program_cost(st.prog);
+        for (const FPhase& ph : st.prog.phases) out[3] += ph.dense;
+    }
+}
 
 int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes) {
     if (passes) *passes = 0;
@@ -614,13 +652,8 @@ int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, 
         if (passes) *passes = (uint32_t)n;
         return QVNT_OK;
     }
-    const unsigned cap = max_tile(r->prec == QVNT_F64);
-    unsigned tq = max_qubits ? max_qubits : kDefaultTile;
-    if (tq > cap) tq = cap;
-    if (tq < (unsigned)kMinFused) tq = kMinFused;
-    if (tq > r->ns) tq = r->ns;
     std::vector<Step> steps;
-    plan(g, n, r->ns, tq, steps, r->dense_min, r->ns >= kSinglePlainQubits, true);
+    compile_fused(g, n, r->ns, r->prec == QVNT_F64, max_qubits, r->dense_min, steps);
     uint32_t count = 0;
     for (const Step& st : steps) {
         if (st.plain) {

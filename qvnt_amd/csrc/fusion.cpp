@@ -1178,7 +1178,58 @@ void parallel_for(size_t n, bool parallel, const F& fn) {
     for (std::thread& t : pool) t.join();
 }
 
+// Fitted run-time model of a compiled queue at 2^30 amplitudes (seconds per
+// pass, seconds per unit of program_cost), per tile size 10, 11, 12: least
+// squares over four measured workloads (30-qubit brickwork per layer and
+// whole, 30-qubit TFIM ring per step and whole, 50 steps), each within 2 %.
+struct TileModel {
+    double pass_s, cost_s;
+};
+constexpr TileModel kModelF64[] = {{5.625e-3, 1.651e-6}, {5.736e-3, 1.847e-6}, {5.728e-3, 2.382e-6}};
+constexpr TileModel kModelF32[] = {{5.147e-3, 0.913e-6}, {4.341e-3, 1.261e-6}, {4.189e-3, 1.553e-6}};
+static_assert(sizeof(kModelF64) / sizeof(kModelF64[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
+static_assert(sizeof(kModelF32) / sizeof(kModelF32[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
+
 }  // namespace
+
+uint64_t program_cost(const Program& prog) {
+    uint64_t cost = 0;
+    for (const FOp& op : prog.ops) cost += op_cost(op);
+    return cost;
+}
+
+unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::vector<Step>& steps, bool single_plain,
+                   bool parallel) {
+    constexpr size_t kTiles = sizeof(kAutoTiles) / sizeof(kAutoTiles[0]);
+    std::vector<Step> cand[kTiles];
+    double est[kTiles];
+    // the candidates compile side by side
+    parallel_for(kTiles, true, [&](size_t k) {
+        est[k] = -1.0;
+        if (kAutoTiles[k] > ns) return;
+        plan(g, n, ns, kAutoTiles[k], cand[k], 0, single_plain, parallel);
+        const TileModel& m = (fp64 ? kModelF64 : kModelF32)[k];
+        double passes = 0.0, cost = 0.0;
+        for (const Step& st : cand[k]) {
+            if (st.plain) {
+                passes += 1.0;
+                continue;
+            }
+            passes += (double)st.prog.passes.size();
+            cost += (double)program_cost(st.prog);
+        }
+        est[k] = passes * m.pass_s + cost * m.cost_s;
+    });
+    size_t best = kTiles;
+    for (size_t k = 0; k < kTiles; ++k)
+        if (est[k] >= 0.0 && (best == kTiles || est[k] < est[best])) best = k;
+    if (best == kTiles) {  // a buffer below every candidate tile
+        plan(g, n, ns, ns, steps, 0, single_plain, parallel);
+        return ns;
+    }
+    steps.swap(cand[best]);
+    return kAutoTiles[best];
+}
 
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min,
           bool single_plain, bool parallel) {

@@ -159,5 +159,24 @@ struct Step {
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0,
           bool single_plain = false, bool parallel = false);
 
+// Sum over the program's ops of the compiler's estimate of vector
+// instructions per thread (arithmetic per output amplitude plus a fixed
+// cost per op).
+uint64_t program_cost(const Program& prog);
+
+// The queue compiled (without dense phases) for each tile size in
+// kAutoTiles that fits the buffer, keeping the steps with the lowest
+// estimated run time: passes x a_t + program_cost x b_t (a plain step
+// counted as a pass), with a_t, b_t per tile size and precision fitted to
+// measured fused applies on MI355X.  Larger tiles need fewer passes;
+// smaller tiles run each op faster (more, smaller blocks per CU hide the
+// barriers between phases).  Returns the tile size kept.
+constexpr unsigned kAutoTiles[] = {10, 11, 12};
+// From 2^28 amplitudes a pass takes milliseconds, more than compiling the
+// candidates costs; single-register applies choose the tile from there.
+constexpr unsigned kAutoTileQubits = 28;
+unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::vector<Step>& steps,
+                   bool single_plain = false, bool parallel = false);
+
 }  // namespace fusion
 }  // namespace qvnt

@@ -161,6 +161,20 @@ int qvnt_reg_apply_fused(qvnt_reg* r, const qvnt_op* op, uint32_t max_qubits, ui
     return apply_fused(r, op->g.data(), op->g.size(), max_qubits, passes);
 }
 
+int qvnt_fused_plan_stats(const qvnt_op* op, uint32_t q_num, qvnt_prec prec, uint32_t max_qubits, uint32_t dense_min,
+                          double* out) {
+    if (!op || !out) return fail(QVNT_E_ARG, "null op or output");
+    if (q_num == 0 || q_num > 40) return fail(QVNT_E_ARG, "q_num must be 1..40");
+    if (prec != QVNT_F64 && prec != QVNT_F32) return fail(QVNT_E_ARG, "unknown precision");
+    const unsigned ns = q_num < 3 ? 3 : q_num;
+    for (const qvnt_gate& g : op->g) {
+        bool noop = false;
+        if (int rc = check_gate(1ull << ns, g, &noop)) return rc;
+    }
+    fused_plan_stats(op->g.data(), op->g.size(), ns, prec == QVNT_F64, max_qubits, dense_min, out);
+    return QVNT_OK;
+}
+
 int qvnt_reg_apply(qvnt_reg* r, const qvnt_op* op) {
     if (!op) return fail(QVNT_E_ARG, "null op");
     return qvnt_reg_apply_gates(r, op->g.data(), op->g.size());

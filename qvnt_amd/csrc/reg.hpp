@@ -84,6 +84,10 @@ int compare(qvnt_reg* a, qvnt_reg* b, double* out);
 
 // fused.hip
 int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes);
+// what apply_fused would compile for a buffer of 2^ns amplitudes (host
+// only): out = {passes, tile qubits, summed op cost, dense phases}
+void fused_plan_stats(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,
+                      double* out);
 namespace fusion { struct Program; }
 size_t program_bytes(const fusion::Program& prog);
 void pack_program(const fusion::Program& prog, unsigned char* host);

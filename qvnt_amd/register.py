@@ -17,6 +17,15 @@ from .ops import MultiOp
 DEFAULT_SEED = 0x51564E54
 
 
+def fused_plan_stats(op, q_num, prec="f64", max_qubits=0, dense_min=0):
+    """What QReg.apply_fused would compile for a register of q_num qubits
+    (host only): passes over the state, the tile size used, the compiler's
+    summed op-cost estimate and the number of dense phases."""
+    out = (ctypes.c_double * 4)()
+    check(lib.qvnt_fused_plan_stats(op.handle, q_num, F64 if prec == "f64" else F32, max_qubits, dense_min, out))
+    return {"passes": int(out[0]), "tile": int(out[1]), "cost": out[2], "dense_phases": int(out[3])}
+
+
 def uniform(seed, stream, index):
     """The seeded draw in [0, 1) (SplitMix64; include/qvnt.h qvnt_uniform)."""
     return lib.qvnt_uniform(seed, stream, index)

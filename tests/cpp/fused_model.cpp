@@ -314,12 +314,22 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
     return true;
 }
 
-static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false) {
+static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false,
+                       bool autotile = false) {
     std::mt19937_64 g(seed);
     const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
-    plan(gates.data(), gates.size(), n, tq, steps, dense, layered);  // layered: single-gate passes go plain
-    {
+    if (autotile) {
+        // the tile size chosen by the run-time model (tq reports it)
+        tq = plan_auto(gates.data(), gates.size(), n, true, steps, layered, true);
+        if (tq != 10 && tq != 11 && tq != 12) {
+            std::printf("n=%u: plan_auto kept tile %u\n", n, tq);
+            ++failures;
+        }
+    } else {
+        plan(gates.data(), gates.size(), n, tq, steps, dense, layered);  // layered: single-gate passes go plain
+    }
+    if (!autotile) {
         // passes compiled on several threads and concatenated: the same program
         std::vector<Step> par;
         plan(gates.data(), gates.size(), n, tq, par, dense, layered, true);
@@ -369,6 +379,7 @@ int main() {
     for (const auto& c : cases) check_case(c[0], c[1], c[2], seed++);
     const unsigned layered[][3] = {{10, 10, 0}, {13, 12, 0}, {14, 11, 0}, {15, 13, 0}, {16, 12, 3}};
     for (const auto& c : layered) check_case(c[0], c[1], c[2], seed++, true);
+    for (unsigned n : {12u, 14u, 16u}) check_case(n, 0, 0, seed++, n == 16, true);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }

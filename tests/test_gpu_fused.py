@@ -164,6 +164,28 @@ def test_fp32_fused_infidelity(orc):
     assert infid <= 1e-5 and err < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_auto_tile_28q(prec):
+    """From 28 qubits, max_qubits = 0 picks the tile size by the run-time
+    model; the program it keeps agrees with the plain form like any other."""
+    n = 28
+    say("auto tile n=%d %s" % (n, prec))
+    specs = sum((circuits.brickwork_layer(n, l) for l in range(6)), [])
+    m = circuits.build(specs, op)
+    with QReg(n, prec=prec) as a, QReg(n, prec=prec) as b, QReg(n, prec=prec) as c:
+        a.apply(m)
+        pb = b.apply_fused(m)
+        pc = c.apply_fused(m, max_qubits=12)
+        err_b, infid_b = a.compare(b)[:2]
+        err_c, infid_c = a.compare(c)[:2]
+        say("passes auto %d, tile 12 %d; max |d| %.2e / %.2e, infidelity %.1e / %.1e"
+            % (pb, pc, err_b, err_c, infid_b, infid_c))
+        if prec == "f64":
+            assert err_b <= TOL and err_c <= TOL
+        else:
+            assert infid_b <= 1e-5 and infid_c <= 1e-5
+
+
 def test_fused_30q_round_trip():
     n = 30
     say("fused round trip n=30")
