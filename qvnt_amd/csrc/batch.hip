@@ -157,10 +157,10 @@ int reset(qvnt_batch* b) {
 }
 
 // Compiles the circuits of registers `sel` with tile cap caps[q] and dense
-// threshold dense[q] (parallel over host threads).  All gates were
-// validated by the caller.
+// threshold dense[q] (parallel over host threads) for amplitudes of
+// amp_bytes.  All gates were validated by the caller.
 void compile_plans(const std::vector<std::vector<qvnt_gate>>& gates, const std::vector<uint32_t>& q,
-                   const std::vector<uint32_t>& sel, const uint32_t* caps, const uint32_t* dense,
+                   const std::vector<uint32_t>& sel, const uint32_t* caps, const uint32_t* dense, unsigned amp_bytes,
                    std::vector<RegPlan>& plans, int* bad) {
     std::atomic<size_t> next{0};
     std::atomic<int> err{-1};
@@ -172,7 +172,7 @@ void compile_plans(const std::vector<std::vector<qvnt_gate>>& gates, const std::
             const unsigned ns = q[k];
             const unsigned tq = std::min<unsigned>(caps[ns], ns);
             std::vector<Step> steps;
-            qvnt::fusion::plan(gates[k].data(), gates[k].size(), ns, tq, steps, dense[ns]);
+            qvnt::fusion::plan(gates[k].data(), gates[k].size(), ns, tq, steps, dense[ns], false, false, amp_bytes);
             RegPlan& rp = plans[k];
             rp.prog = Program();
             // an all-fusable queue compiles to one program; a gate the fused
@@ -419,7 +419,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         //     if the whole batch gets faster.
         auto time_all = [&](const uint32_t* tc, const uint32_t* td, const std::vector<uint32_t>& recompile,
                             double* ms) -> int {
-            compile_plans(gates, b->q, recompile, tc, td, plans, &bad);
+            compile_plans(gates, b->q, recompile, tc, td, (unsigned)b->amp(), plans, &bad);
             if (bad >= 0) return no_batched_form();
             Schedule S;
             if (int rc = build_schedule(b, plans, all, S)) return rc;
@@ -448,7 +448,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
                 }
             }
         // plans must match caps/dense again before the per-size trials
-        compile_plans(gates, b->q, all, caps, dense, plans, &bad);
+        compile_plans(gates, b->q, all, caps, dense, (unsigned)b->amp(), plans, &bad);
         if (bad >= 0) return no_batched_form();
         std::map<uint32_t, std::vector<uint32_t>> by_q;
         for (uint32_t k = 0; k < b->count; ++k) by_q[b->q[k]].push_back(k);
@@ -464,12 +464,12 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
                 best = ms;
                 caps[n] = n;
             } else {
-                compile_plans(gates, b->q, kv.second, caps, dense, plans, &bad);  // back to the kept plans
+                compile_plans(gates, b->q, kv.second, caps, dense, (unsigned)b->amp(), plans, &bad);  // back to the kept plans
             }
         }
         if (int rc = reset(b)) return rc;
     }
-    compile_plans(gates, b->q, all, caps, dense, plans, &bad);
+    compile_plans(gates, b->q, all, caps, dense, (unsigned)b->amp(), plans, &bad);
     if (bad >= 0) return no_batched_form();
     if (int rc = build_schedule(b, plans, all, b->sched)) return rc;
     std::memcpy(b->caps, caps, sizeof(caps));

@@ -618,7 +618,7 @@ unsigned compile_fused(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uin
     if (tq > cap) tq = cap;
     if (tq < (unsigned)kMinFused) tq = kMinFused;
     if (tq > ns) tq = ns;
-    plan(g, n, ns, tq, steps, dense_min, single_plain, true);
+    plan(g, n, ns, tq, steps, dense_min, single_plain, true, fp64 ? 16 : 8);
     return tq;
 }
 

@@ -264,15 +264,17 @@ struct Left {
 };
 
 // Qubits 0..row_bits-1 belong to every pass, so each tile row is that many
-// consecutive amplitudes in HBM (3: 128-byte lines in FP64).  QVNT_ROW_BITS
-// (1..4) overrides it for tuning.
-unsigned row_bits() {
-    static const unsigned r = [] {
+// consecutive amplitudes in HBM: one 128-byte cache line (3 bits in FP64, 4
+// in FP32).  A shorter row would leave the rest of each line to another
+// tile, fetched again later.  QVNT_ROW_BITS (1..4) overrides it for tuning.
+unsigned row_bits(unsigned amp_bytes) {
+    static const int env = [] {
         const char* s = std::getenv("QVNT_ROW_BITS");
-        const int v = s ? std::atoi(s) : 3;
-        return (unsigned)(v >= 1 && v <= 4 ? v : 3);
+        const int v = s ? std::atoi(s) : 0;
+        return v >= 1 && v <= 4 ? v : 0;
     }();
-    return r;
+    if (env) return (unsigned)env;
+    return amp_bytes <= 8 ? 4u : 3u;
 }
 
 // Greedy pass formation (see the file comment).  A second scan with the
@@ -280,9 +282,9 @@ unsigned row_bits() {
 // touches no qubit of T (in this pass it would only scale whole tiles; in
 // a later one its qubits may be registers), provided the pass keeps some
 // matrix primitive, so every pass makes progress.
-std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns, unsigned tq) {
+std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns, unsigned tq, unsigned rows) {
     std::vector<PassPlan> passes;
-    const unsigned rb = ns < row_bits() ? ns : row_bits();
+    const unsigned rb = ns < rows ? ns : rows;
     const uint64_t forced = (1ull << rb) - 1u;
     // Once the left-behind matrix primitives target every qubit the queue
     // touches, every later primitive is blocked: a scan stops there.
@@ -1186,7 +1188,7 @@ struct TileModel {
     double pass_s, cost_s;
 };
 constexpr TileModel kModelF64[] = {{5.625e-3, 1.651e-6}, {5.736e-3, 1.847e-6}, {5.728e-3, 2.382e-6}};
-constexpr TileModel kModelF32[] = {{5.147e-3, 0.913e-6}, {4.341e-3, 1.261e-6}, {4.189e-3, 1.553e-6}};
+constexpr TileModel kModelF32[] = {{2.274e-3, 1.735e-6}, {2.081e-3, 1.855e-6}, {2.317e-3, 1.929e-6}};
 static_assert(sizeof(kModelF64) / sizeof(kModelF64[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 static_assert(sizeof(kModelF32) / sizeof(kModelF32[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 
@@ -1207,7 +1209,7 @@ unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::ve
     parallel_for(kTiles, true, [&](size_t k) {
         est[k] = -1.0;
         if (kAutoTiles[k] > ns) return;
-        plan(g, n, ns, kAutoTiles[k], cand[k], 0, single_plain, parallel);
+        plan(g, n, ns, kAutoTiles[k], cand[k], 0, single_plain, parallel, fp64 ? 16 : 8);
         const TileModel& m = (fp64 ? kModelF64 : kModelF32)[k];
         double passes = 0.0, cost = 0.0;
         for (const Step& st : cand[k]) {
@@ -1224,7 +1226,7 @@ unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::ve
     for (size_t k = 0; k < kTiles; ++k)
         if (est[k] >= 0.0 && (best == kTiles || est[k] < est[best])) best = k;
     if (best == kTiles) {  // a buffer below every candidate tile
-        plan(g, n, ns, ns, steps, 0, single_plain, parallel);
+        plan(g, n, ns, ns, steps, 0, single_plain, parallel, fp64 ? 16 : 8);
         return ns;
     }
     steps.swap(cand[best]);
@@ -1232,7 +1234,7 @@ unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::ve
 }
 
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min,
-          bool single_plain, bool parallel) {
+          bool single_plain, bool parallel, unsigned amp_bytes) {
     const uint64_t lmask = (1ull << ns) - 1u;
     size_t i = 0;
     while (i < n) {
@@ -1246,7 +1248,7 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
             for (size_t k = before; k < prims.size(); ++k) prims[k].src = (long)j;
         }
         if (!prims.empty()) {
-            const std::vector<PassPlan> passes = schedule_passes(merge_single(prims), ns, tq);
+            const std::vector<PassPlan> passes = schedule_passes(merge_single(prims), ns, tq, row_bits(amp_bytes));
             // primitives per source SingleOp, to find passes that hold one
             // whole SingleOp and nothing else: one plain launch moves at
             // most the same bytes as a pass (fewer with controls) and skips

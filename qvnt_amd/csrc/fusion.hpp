@@ -156,8 +156,10 @@ struct Step {
 // nothing else becomes a plain step for that SingleOp.  With parallel, the
 // passes of a long queue compile on several host threads (the output is
 // the same); callers that already compile many queues at once leave it off.
+// amp_bytes (16: FP64, 8: FP32) sets how many low qubits every pass holds,
+// so that each tile row is one 128-byte line.
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min = 0,
-          bool single_plain = false, bool parallel = false);
+          bool single_plain = false, bool parallel = false, unsigned amp_bytes = 16);
 
 // Sum over the program's ops of the compiler's estimate of vector
 // instructions per thread (arithmetic per output amplitude plus a fixed

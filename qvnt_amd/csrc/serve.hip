@@ -367,7 +367,7 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
         unsigned tq = q->max_qubits ? q->max_qubits : fusion::kDefaultTile;
         tq = std::min(std::max(tq, (unsigned)fusion::kMinFused), std::min(cap, ns));
         fusion::plan(rec->gates.data(), rec->gates.size(), ns, tq, rec->steps, q->dense_min,
-                     ns >= fusion::kSinglePlainQubits, true);
+                     ns >= fusion::kSinglePlainQubits, true, q->prec == QVNT_F64 ? 16 : 8);
     } else {
         for (const qvnt_gate& g : rec->gates) {
             fusion::Step st;

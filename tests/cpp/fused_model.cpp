@@ -315,7 +315,7 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
 }
 
 static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false,
-                       bool autotile = false) {
+                       bool autotile = false, unsigned amp_bytes = 16) {
     std::mt19937_64 g(seed);
     const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
@@ -327,12 +327,13 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
             ++failures;
         }
     } else {
-        plan(gates.data(), gates.size(), n, tq, steps, dense, layered);  // layered: single-gate passes go plain
+        // layered: single-gate passes go plain; amp_bytes 8: the FP32 tile rows
+        plan(gates.data(), gates.size(), n, tq, steps, dense, layered, false, amp_bytes);
     }
     if (!autotile) {
         // passes compiled on several threads and concatenated: the same program
         std::vector<Step> par;
-        plan(gates.data(), gates.size(), n, tq, par, dense, layered, true);
+        plan(gates.data(), gates.size(), n, tq, par, dense, layered, true, amp_bytes);
         if (!same_steps(steps, par)) {
             std::printf("n=%u tq=%u dense=%u: the parallel compile differs\n", n, tq, dense);
             ++failures;
@@ -380,6 +381,7 @@ int main() {
     const unsigned layered[][3] = {{10, 10, 0}, {13, 12, 0}, {14, 11, 0}, {15, 13, 0}, {16, 12, 3}};
     for (const auto& c : layered) check_case(c[0], c[1], c[2], seed++, true);
     for (unsigned n : {12u, 14u, 16u}) check_case(n, 0, 0, seed++, n == 16, true);
+    for (unsigned n : {9u, 13u, 15u}) check_case(n, 11, n == 13 ? 2 : 0, seed++, false, false, 8);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
