@@ -266,12 +266,12 @@ struct Left {
 // Qubits 0..row_bits-1 belong to every pass, so each tile row is that many
 // consecutive amplitudes in HBM: one 128-byte cache line (3 bits in FP64, 4
 // in FP32).  A shorter row would leave the rest of each line to another
-// tile, fetched again later.  QVNT_ROW_BITS (1..4) overrides it for tuning.
+// tile, fetched again later.  QVNT_ROW_BITS (1..6) overrides it for tuning.
 unsigned row_bits(unsigned amp_bytes) {
     static const int env = [] {
         const char* s = std::getenv("QVNT_ROW_BITS");
         const int v = s ? std::atoi(s) : 0;
-        return v >= 1 && v <= 4 ? v : 0;
+        return v >= 1 && v <= 6 ? v : 0;
     }();
     if (env) return (unsigned)env;
     return amp_bytes <= 8 ? 4u : 3u;
