@@ -282,6 +282,26 @@ def test_large_register_round_trip(n):
         assert all(int(o) == start for o in outs)
 
 
+@pytest.mark.parametrize("n,prec", [(34, "f64"), (35, "f32")])
+def test_largest_register(n, prec):
+    """The largest register one MI355X holds (256 GiB of amplitudes): the
+    plain and fused forms on the top and bottom qubits, with outcomes fixed
+    by the mathematics (x moves the basis state; h then its inverse returns
+    it; a controlled x fires only where the control is set)."""
+    say("largest register n=%d %s" % (n, prec))
+    start = 0x155555555 & ((1 << n) - 1)
+    top = 1 << (n - 1)
+    with QReg.with_state(n, start, prec=prec) as r:
+        r.apply(op.x(top))                       # plain: flip the top qubit
+        r.apply(op.h(1) * op.h(1))               # h h = identity (exact to rounding)
+        r.apply(op.x(2).c(top))                  # control set: fires
+        r.apply_fused(op.x(1).c(top) * op.rz(0.7, top) * op.ry(0.4, 1 << 5) * op.ry(-0.4, 1 << 5))
+        want = start ^ top ^ 2 ^ 1
+        assert abs(r.get_absolute() - 1.0) < (1e-12 if prec == "f64" else 1e-5)
+        outs = r.sample_mask((1 << n) - 1, [qvnt_amd.uniform(5, n, k) for k in range(8)])
+        assert all(int(o) == want for o in outs), [hex(int(o)) for o in outs]
+
+
 def test_compare_two_registers():
     """qvnt_reg_compare against numpy on the copied-out states (FP64 vs FP32)."""
     n = 12
