@@ -47,12 +47,20 @@ __device__ __forceinline__ unsigned dpopc(uint64_t x) { return (unsigned)__popcl
 // Fixed bits of a launch: `zeros` holds their positions, `set` the ones
 // among them that are 1, n = popcount(zeros).  The walk over `zeros` is
 // uniform scalar work (no table loads in the kernels' address path).
+// `skew` (kernels with two units per thread): 0, or S >= 1 for a thread
+// whose two units are 2^S apart in the compact index (see unit_index).
 struct Deposit {
     uint64_t zeros;
     uint64_t set;
     uint32_t n;
-    uint32_t pad;
+    uint32_t skew;
 };
+
+// Compact index of unit u (0 or 1) of thread tau when the units sit 2^S
+// apart: bit S of the index is u, the bits of tau close around it.
+__device__ __forceinline__ uint64_t unit_index(uint64_t tau, uint32_t u, uint32_t S) {
+    return ((tau >> S) << (S + 1)) | ((uint64_t)u << S) | (tau & ((1ull << S) - 1u));
+}
 
 __device__ __forceinline__ uint64_t deposit(uint64_t g, const Deposit& d) {
     for (uint64_t m = d.zeros; m; m &= m - 1) {
