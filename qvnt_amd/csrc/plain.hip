@@ -215,11 +215,15 @@ int unroll_setting() {
     return u;
 }
 // Diagonal and pair kernels with two units per thread place them 2^S apart
-// in the compact index: two streams 128 KiB apart run faster on MI355X's
-// HBM than one (measured at 30 qubits FP64: k_diag 6.07 -> 6.39 TB/s,
-// k_pair 5.68 -> 5.88 TB/s).  QVNT_DIAG_SKEW / QVNT_PAIR_SKEW override S
-// (0: the units blockDim apart).
-uint32_t skew_setting(GClass cls, size_t amp_bytes) {
+// in the compact index: two streams 128 KiB apart.  Measured in FP64:
+// k_diag gains 3-5 % from 28 to 31 qubits (6.08 -> 6.35 TB/s at 30), but
+// above kSkewMaxBytes one stream is as fast or faster (34 qubits: 6.13
+// against 5.62 TB/s), so diagonals of larger states keep one; k_pair gains
+// at 30 qubits (5.71 -> 6.04 TB/s) and loses nothing at 34.
+// QVNT_DIAG_SKEW / QVNT_PAIR_SKEW override S at every size (0: the units
+// blockDim apart).
+constexpr uint64_t kSkewMaxBytes = 32ull << 30;
+uint32_t skew_setting(GClass cls, size_t amp_bytes, uint64_t state_bytes) {
     static const int diag_env = [] {
         const char* s = std::getenv("QVNT_DIAG_SKEW");
         return s ? std::atoi(s) : -1;
@@ -231,6 +235,7 @@ uint32_t skew_setting(GClass cls, size_t amp_bytes) {
     const int env = cls == GClass::Diag ? diag_env : cls == GClass::Pair ? pair_env : 0;
     if (env >= 0) return env <= 40 ? (uint32_t)env : 0u;
     if (cls != GClass::Diag && cls != GClass::Pair) return 0u;
+    if (cls == GClass::Diag && state_bytes > kSkewMaxBytes) return 0u;
     return amp_bytes == 16 ? 13u : 14u;  // 128 KiB
 }
 
@@ -265,7 +270,7 @@ Par<R> make_par(const qvnt_gate& g) {
 template <class R, int U, bool NT>
 int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d0) {
     Deposit d = d0;
-    const uint32_t S = U == 2 ? skew_setting(cls, sizeof(cplx<R>)) : 0u;
+    const uint32_t S = U == 2 ? skew_setting(cls, sizeof(cplx<R>), r->len * sizeof(cplx<R>)) : 0u;
     d.skew = (S && groups >= (2ull << S)) ? S : 0u;
     const Par<R> p = make_par<R>(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
