@@ -504,7 +504,8 @@ def main():
     ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
-    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent requests on the device")
+    ap.add_argument("--slots", type=int, default=4,
+                    help="serve: concurrent requests on the device (one stream each; the runtime has 4 hardware queues)")
     ap.add_argument("--arena-gib", type=float, default=160.0, help="serve: HBM admission budget")
     ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
     ap.add_argument("--fused-steps", type=int, default=50, help="tfim: Trotter steps through the fused form")

@@ -267,7 +267,10 @@ typedef struct qvnt_result {
     double total;             /* |psi|^2 of the final state                    */
     double t_submit, t_admit, t_done;  /* seconds on the server clock         */
 } qvnt_result;
-/* hbm_bytes: the arena (admission budget); slots: concurrent requests. */
+/* hbm_bytes: the arena (admission budget); slots: concurrent requests, one
+ * HIP stream each.  More slots than the runtime's hardware queues (4 by
+ * default) put several streams on one queue, where a small request's
+ * kernels wait behind a large one's: 4 slots measured best. */
 int qvnt_server_new(uint64_t hbm_bytes, uint32_t slots, qvnt_server** out);
 int qvnt_server_submit(qvnt_server* s, const qvnt_request* req);
 /* Up to cap finished results; waits up to wait_ms for the first one. */

@@ -41,7 +41,7 @@ class Server:
     """Requests are admitted as slots and arena space free up; outcomes
     equal those of the same request run alone."""
 
-    def __init__(self, hbm_bytes, slots=8):
+    def __init__(self, hbm_bytes, slots=4):
         h = ctypes.c_void_p()
         check(lib.qvnt_server_new(hbm_bytes, slots, ctypes.byref(h)))
         self._h = h
