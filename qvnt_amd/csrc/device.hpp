@@ -56,10 +56,11 @@ struct Deposit {
     uint32_t skew;
 };
 
-// Compact index of unit u (0 or 1) of thread tau when the units sit 2^S
-// apart: bit S of the index is u, the bits of tau close around it.
-__device__ __forceinline__ uint64_t unit_index(uint64_t tau, uint32_t u, uint32_t S) {
-    return ((tau >> S) << (S + 1)) | ((uint64_t)u << S) | (tau & ((1ull << S) - 1u));
+// Compact index of unit u (0 .. 2^ubits - 1) of thread tau when the units
+// sit 2^S apart: bits S .. S + ubits - 1 of the index are u, the bits of tau
+// close around them.
+__device__ __forceinline__ uint64_t unit_index(uint64_t tau, uint32_t u, uint32_t S, uint32_t ubits = 1) {
+    return ((tau >> S) << (S + ubits)) | ((uint64_t)u << S) | (tau & ((1ull << S) - 1u));
 }
 
 __device__ __forceinline__ uint64_t deposit(uint64_t g, const Deposit& d) {

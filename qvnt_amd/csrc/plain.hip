@@ -159,18 +159,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair_lanes(cplx<R>* __restrict__ 
         const uint64_t base = chunk * per + threadIdx.x;
         cplx<R> v[U];
         uint64_t idx[U];
+        bool in[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint64_t t = base + (uint64_t)u * B;
+            // units 2^skew apart (skew >= 6 keeps the lane bits below them)
+            const uint64_t t = (U == 4 && dep.skew) ? unit_index(chunk * B + threadIdx.x, (uint32_t)u, dep.skew, 2)
+                                                    : base + (uint64_t)u * B;
             idx[u] = deposit(t, dep);
+            in[u] = t < groups;
             v[u] = cplx<R>{(R)0, (R)0};
-            if (t < groups) v[u] = ld<true>(psi + idx[u]);
+            if (in[u]) v[u] = ld<true>(psi + idx[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const cplx<R> other{__shfl_xor(v[u].re, lm), __shfl_xor(v[u].im, lm)};
-            const uint64_t t = base + (uint64_t)u * B;
-            if (t < groups) st<true>(psi + idx[u], f_pair<K>(g, idx[u], v[u], other));
+            if (in[u]) st<true>(psi + idx[u], f_pair<K>(g, idx[u], v[u], other));
         }
     }
 }
@@ -237,6 +240,21 @@ uint32_t skew_setting(GClass cls, size_t amp_bytes, uint64_t state_bytes) {
     if (cls != GClass::Diag && cls != GClass::Pair) return 0u;
     if (cls == GClass::Diag && state_bytes > kSkewMaxBytes) return 0u;
     return amp_bytes == 16 ? 13u : 14u;  // 128 KiB
+}
+
+// The lane kernel's four units per lane 2^12 apart in the compact index
+// (measured, ry on qubits 0-5: FP64 at 30 qubits 5.84 TB/s one block apart,
+// 6.35 at 2^12, 5.97 at 2^13, 5.86 at 2^14; FP32 at 31 qubits 5.58, 6.57,
+// 6.36, 5.91); like the diagonals, only up to kSkewMaxBytes of state.
+// QVNT_LANES_SKEW overrides S (0: one block apart).
+uint32_t lanes_skew_setting(size_t amp_bytes, uint64_t state_bytes) {
+    static const int env = [] {
+        const char* s = std::getenv("QVNT_LANES_SKEW");
+        return s ? std::atoi(s) : -1;
+    }();
+    if (env >= 0) return env <= 40 ? (uint32_t)env : 0u;
+    (void)amp_bytes;
+    return state_bytes > kSkewMaxBytes ? 0u : 12u;
 }
 
 unsigned block_setting() {
@@ -335,6 +353,9 @@ int launch_lanes(qvnt_reg* r, const qvnt_gate& gate) {
     const unsigned B = block_setting();
     constexpr int U = 4;  // four amplitudes per lane: as many loads in flight as k_pair's two orbits
     const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+    // the four units of a lane 2^S apart, as the pair kernels' two orbits
+    const uint32_t S = lanes_skew_setting(sizeof(cplx<R>), r->len * sizeof(cplx<R>));
+    d.skew = (S >= 6 && groups >= (4ull << S)) ? S : 0u;
     const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid)), block(B);
     hipStream_t s = r->stream;
 #define QVNT_LANES(KIND)                                                                             \
