@@ -604,6 +604,8 @@ int run_program(qvnt_reg* r, const Program& prog) {
     return launch_program(r, prog, r->prog);
 }
 
+}  // namespace
+
 // The steps of a fused apply on 2^ns (>= 2^kMinFused) amplitudes; returns
 // the tile size.  max_qubits = 0 and no dense phases from kAutoTileQubits:
 // the tile the run-time model picks; otherwise max_qubits (0: the default)
@@ -621,8 +623,6 @@ unsigned compile_fused(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uin
     plan(g, n, ns, tq, steps, dense_min, single_plain, true, fp64 ? 16 : 8);
     return tq;
 }
-
-}  // namespace
 
 void fused_plan_stats(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,
                       double* out) {

@@ -84,6 +84,13 @@ int compare(qvnt_reg* a, qvnt_reg* b, double* out);
 
 // fused.hip
 int apply_fused(qvnt_reg* r, const qvnt_gate* g, size_t n, uint32_t max_qubits, uint32_t* passes);
+// the steps of a fused apply on 2^ns (>= 2^kMinFused) amplitudes (host
+// only; returns the tile size): the tile the run-time model picks when
+// max_qubits = 0 and dense_min = 0 from 28 qubits, else max_qubits (0: the
+// default) clamped to the precision's LDS tile and the buffer
+namespace fusion { struct Step; }
+unsigned compile_fused(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,
+                       std::vector<fusion::Step>& steps);
 // what apply_fused would compile for a buffer of 2^ns amplitudes (host
 // only): out = {passes, tile qubits, summed op cost, dense phases}
 void fused_plan_stats(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, uint32_t max_qubits, uint32_t dense_min,

@@ -363,11 +363,9 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
         }
     const unsigned ns = q->q_num < 3 ? 3 : q->q_num;
     if (q->fused && ns >= (unsigned)fusion::kMinFused) {
-        const unsigned cap = fusion::max_tile(q->prec == QVNT_F64);
-        unsigned tq = q->max_qubits ? q->max_qubits : fusion::kDefaultTile;
-        tq = std::min(std::max(tq, (unsigned)fusion::kMinFused), std::min(cap, ns));
-        fusion::plan(rec->gates.data(), rec->gates.size(), ns, tq, rec->steps, q->dense_min,
-                     ns >= fusion::kSinglePlainQubits, true, q->prec == QVNT_F64 ? 16 : 8);
+        // the same program qvnt_reg_apply_fused would run
+        compile_fused(rec->gates.data(), rec->gates.size(), ns, q->prec == QVNT_F64, q->max_qubits, q->dense_min,
+                      rec->steps);
     } else {
         for (const qvnt_gate& g : rec->gates) {
             fusion::Step st;
