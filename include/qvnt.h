@@ -272,6 +272,8 @@ typedef struct qvnt_result {
  * default) put several streams on one queue, where a small request's
  * kernels wait behind a large one's: 4 slots measured best. */
 int qvnt_server_new(uint64_t hbm_bytes, uint32_t slots, qvnt_server** out);
+/* QVNT_E_ARG if req->id belongs to a request submitted earlier and not yet
+ * returned by qvnt_server_poll (its outcome buffer is still being filled). */
 int qvnt_server_submit(qvnt_server* s, const qvnt_request* req);
 /* Up to cap finished results; waits up to wait_ms for the first one. */
 int qvnt_server_poll(qvnt_server* s, qvnt_result* out, uint32_t cap, uint32_t* n, int wait_ms);

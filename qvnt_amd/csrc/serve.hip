@@ -26,6 +26,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 #include "fusion.hpp"
@@ -131,6 +132,7 @@ struct qvnt_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Record*> pending;
     std::deque<Record*> done;
+    std::unordered_set<uint64_t> ids;  // ids submitted and not yet returned by poll
     std::thread worker;
     bool stop = false;
     int device = 0;
@@ -381,6 +383,12 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
                  kAlign + rec->prog_bytes;
     rec->res.id = q->id;
     std::lock_guard<std::mutex> lk(s->mu);
+    // the caller's outcome buffer is written when the request finishes, and
+    // poll returns results by id: a second request under a live id is refused
+    if (!s->ids.insert(q->id).second) {
+        delete rec;
+        return fail(QVNT_E_ARG, "request id " + std::to_string(q->id) + " is already in flight");
+    }
     rec->res.t_submit = t_in;
     if (rec->bytes > s->arena.size()) {
         rec->res.status = QVNT_E_OOM;
@@ -403,6 +411,7 @@ int qvnt_server_poll(qvnt_server* s, qvnt_result* out, uint32_t cap, uint32_t* n
         Record* rec = s->done.front();
         s->done.pop_front();
         out[k++] = rec->res;
+        s->ids.erase(rec->res.id);
         delete rec;
     }
     *n = k;

@@ -50,6 +50,9 @@ class Server:
     def submit(self, rid, q_num, op, mask, draws, fused=True, prec="f64", init_state=0, max_qubits=0, dense_min=0):
         if not isinstance(op, MultiOp):
             raise TypeError("submit expects a MultiOp")
+        if rid in self._live:
+            # the first request's outcome buffer must stay alive until it is polled
+            raise ValueError("request id %d is already in flight" % rid)
         draws = np.ascontiguousarray(draws, np.float64)
         outs = np.zeros(len(draws), np.uint64)
         req = Request(rid, q_num, {"f64": F64, "f32": F32}[prec], init_state, op.handle, 1 if fused else 0,

@@ -96,3 +96,27 @@ def test_admission_by_budget():
     assert waited, "every request was admitted at once although the arena holds two 20-qubit states"
     for k in res:
         assert res[k]["t_submit"] <= res[k]["t_admit"] <= res[k]["t_done"]
+
+
+def test_duplicate_live_id_is_refused():
+    import ctypes
+    from qvnt_amd._lib import E_ARG, lib
+    from qvnt_amd.serve import Request
+    o = op.h(0b111)
+    draws = np.array([0.25, 0.75])
+    with Server(1 << 28, slots=2) as s:
+        s.submit(5, 3, o, 0b111, draws)
+        # the Python side refuses it ...
+        with pytest.raises(ValueError):
+            s.submit(5, 3, o, 0b111, draws)
+        # ... and so does the C side, for a caller that binds it directly
+        outs = np.zeros(2, np.uint64)
+        req = Request(5, 3, 0, 0, o.handle, 1, 0, 0, 0, 0b111, 2, draws.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      outs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        assert lib.qvnt_server_submit(s._h, ctypes.byref(req)) == E_ARG
+        res = s.drain(1)
+        assert len(res) == 1 and res[0]["id"] == 5 and res[0]["status"] == 0
+        # once polled, the id may be used again
+        s.submit(5, 3, o, 0b111, draws)
+        res = s.drain(1)
+        assert res[0]["status"] == 0
