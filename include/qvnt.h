@@ -220,6 +220,16 @@ int qvnt_reg_measure(qvnt_reg* reg, double u, uint32_t flags, uint64_t* out);
 /* Non-destructive shots: out[k] = r_k & mask for draws u[k], state untouched
  * (each equals measure_mask on a copy of the register). */
 int qvnt_reg_sample_mask(qvnt_reg* reg, uint64_t mask, const double* u, size_t shots, uint64_t* out);
+/* qvnt_reg_sample_mask without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same outcomes as qvnt_reg_sample_mask for the same draws.  The measurement
+ * tree, draws and outcomes live in one fixed 16 MiB device workspace per GPU,
+ * held until the process ends: registers up to 30 qubits (fewer shots at 30);
+ * a call that needs more returns QVNT_E_ARG.  All calls on one GPU share that
+ * workspace: calls that overlap on one GPU (other threads, other registers)
+ * race.  Synchronises the register's stream. */
+int qvnt_reg_sample_mask_noalloc(qvnt_reg* reg, uint64_t mask, const double* u, size_t shots, uint64_t* out);
 
 /* Device-side comparison of two registers of equal size (any precisions;
  * no reference counterpart): out = { Re<a|b>, Im<a|b>, |a|^2, |b|^2,

@@ -93,6 +93,7 @@ def _declare(L):
         "qvnt_reg_measure_mask": ([vp, u64, d, u32, u64p], i),
         "qvnt_reg_measure": ([vp, d, u32, u64p], i),
         "qvnt_reg_sample_mask": ([vp, u64, dp, sz, u64p], i),
+        "qvnt_reg_sample_mask_noalloc": ([vp, u64, dp, sz, u64p], i),
         "qvnt_reg_compare": ([vp, vp, dp], i),
         "qvnt_reg_get_state": ([vp, vp, sz], i),
         "qvnt_reg_set_state": ([vp, vp, sz], i),

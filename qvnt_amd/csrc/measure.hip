@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "device.hpp"
@@ -259,30 +261,53 @@ int ensure_workspace(qvnt_reg* r, size_t tree_doubles, size_t shots) {
     return QVNT_OK;
 }
 
+// The tree levels of the first `count` weights into `tree` (laid out by
+// tree_info(count)); timed: bracket the launches with the register's timing
+// events (the allocation-free path passes false: it creates no events).
 template <class R>
-int build_levels(qvnt_reg* r, uint64_t count, const TreeInfo& info) {
+int build_levels(qvnt_reg* r, double* tree, uint64_t count, const TreeInfo& info, bool timed) {
     const cplx<R>* psi = static_cast<const cplx<R>*>(r->psi);
-    Timed t(r, QVNT_K_REDUCE, (double)r->amp_bytes() * (double)count);
+    qvnt_reg* tr = timed ? r : nullptr;
+    Timed t(tr, QVNT_K_REDUCE, (double)r->amp_bytes() * (double)count);
     if (info.top < kLeafLog) {
-        hipLaunchKernelGGL(k_small<R>, dim3(1), dim3(kBlock), 0, r->stream, psi, (unsigned)count, r->tree + info.off[info.top]);
+        hipLaunchKernelGGL(k_small<R>, dim3(1), dim3(kBlock), 0, r->stream, psi, (unsigned)count, tree + info.off[info.top]);
         QVNT_HIP(hipGetLastError());
         return QVNT_OK;
     }
     const uint64_t nleaves = count >> kLeafLog;
     hipLaunchKernelGGL(k_leaf<R>, dim3((unsigned)(nleaves < kMaxGrid ? nleaves : kMaxGrid)), dim3(kBlock), 0, r->stream, psi,
-                       nleaves, r->tree + info.off[kLeafLog]);
+                       nleaves, tree + info.off[kLeafLog]);
     QVNT_HIP(hipGetLastError());
     for (uint32_t L = kLeafLog; L < info.top;) {
         const uint32_t steps = (info.top - L) < kLeafLog ? (info.top - L) : kLeafLog;
         const unsigned span = 1u << steps;
         const uint64_t n_in = count >> L;
         LevelPtrs lp{};
-        for (uint32_t k = 1; k <= steps; ++k) lp.p[k] = r->tree + info.off[L + k];
+        for (uint32_t k = 1; k <= steps; ++k) lp.p[k] = tree + info.off[L + k];
         hipLaunchKernelGGL(k_upper, dim3((unsigned)(n_in / span)), dim3(kBlock), 0, r->stream,
-                           (const double*)(r->tree + info.off[L]), span, lp);
+                           (const double*)(tree + info.off[L]), span, lp);
         QVNT_HIP(hipGetLastError());
         L += steps;
     }
+    return QVNT_OK;
+}
+
+// Tree levels plus one draw per u_dev[k] into out_dev, total into
+// total_dev[0]; every pointer is device memory; no synchronisation.
+int sample_into(qvnt_reg* r, double* tree, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev,
+                double* total_dev, bool timed) {
+    const TreeInfo info = tree_info(count);
+    int rc = r->prec == QVNT_F64 ? build_levels<double>(r, tree, count, info, timed)
+                                 : build_levels<float>(r, tree, count, info, timed);
+    if (rc) return rc;
+    Timed t(timed ? r : nullptr, QVNT_K_OTHER, 0.0);
+    if (r->prec == QVNT_F64)
+        hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                           static_cast<const c64*>(r->psi), (const double*)tree, info, u_dev, out_dev, total_dev);
+    else
+        hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
+                           static_cast<const c32*>(r->psi), (const double*)tree, info, u_dev, out_dev, total_dev);
+    QVNT_HIP(hipGetLastError());
     return QVNT_OK;
 }
 
@@ -292,7 +317,8 @@ int tree_build(qvnt_reg* r, uint64_t count, double* total_out) {
     const TreeInfo info = tree_info(count);
     int rc = ensure_workspace(r, (size_t)(info.off[info.top] + 1), 1);
     if (rc) return rc;
-    rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
+    rc = r->prec == QVNT_F64 ? build_levels<double>(r, r->tree, count, info, true)
+                             : build_levels<float>(r, r->tree, count, info, true);
     if (rc) return rc;
     if (total_out) {
         QVNT_HIP(hipMemcpyAsync(total_out, r->tree + info.off[info.top], sizeof(double), hipMemcpyDeviceToHost, r->stream));
@@ -307,19 +333,8 @@ size_t tree_doubles(uint64_t count) {
 }
 
 int sample_launch(qvnt_reg* r, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev, double* total_dev) {
-    const TreeInfo info = tree_info(count);
-    if ((size_t)(info.off[info.top] + 1) > r->tree_cap) return fail(QVNT_E_ARG, "measurement tree workspace too small");
-    int rc = r->prec == QVNT_F64 ? build_levels<double>(r, count, info) : build_levels<float>(r, count, info);
-    if (rc) return rc;
-    Timed t(r, QVNT_K_OTHER, 0.0);
-    if (r->prec == QVNT_F64)
-        hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                           static_cast<const c64*>(r->psi), (const double*)r->tree, info, u_dev, out_dev, total_dev);
-    else
-        hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                           static_cast<const c32*>(r->psi), (const double*)r->tree, info, u_dev, out_dev, total_dev);
-    QVNT_HIP(hipGetLastError());
-    return QVNT_OK;
+    if (tree_doubles(count) > r->tree_cap) return fail(QVNT_E_ARG, "measurement tree workspace too small");
+    return sample_into(r, r->tree, count, u_dev, shots, out_dev, total_dev, true);
 }
 
 int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out) {
@@ -428,3 +443,87 @@ int compare(qvnt_reg* a, qvnt_reg* b, double* out) {
 }
 
 }  // namespace qvnt
+
+// ------------------------------------------------ allocation-free shots
+// qvnt_reg_sample_mask_noalloc: the shots of qvnt_reg_sample_mask with the
+// tree levels, draws, outcomes and total in `scratch`, a file-scope device
+// array of kScratchBytes that every GPU the library runs on holds until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current and kept in g_scratch.  Every
+// call on one GPU shares that GPU's array, so calls that overlap on one GPU
+// race.  Nothing in it needs to start at zero: each call writes the levels,
+// the draws, every outcome and the total before reading them.
+namespace qvnt {
+namespace {
+
+constexpr size_t kScratchBytes = size_t(16) << 20;  // 16 MiB: registers up to 30 qubits
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_address(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(QVNT_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        int cur = 0;
+        QVNT_HIP(hipGetDevice(&cur));
+        if (cur != device) QVNT_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(QVNT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch[device]);
+    return QVNT_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+}  // namespace qvnt
+
+extern "C" int qvnt_reg_sample_mask_noalloc(qvnt_reg* r, uint64_t mask, const double* u, size_t shots, uint64_t* out) {
+    using namespace qvnt;
+    if (!r) return fail(QVNT_E_ARG, "null register");
+    if (shots == 0) return QVNT_OK;
+    if (!u || !out) return fail(QVNT_E_ARG, "null draws or outcomes");
+    if (shots > (1u << 20)) return fail(QVNT_E_ARG, "at most 2^20 shots per call");
+    mask &= (1ull << r->q_num) - 1u;
+    if (mask == 0) {
+        for (size_t k = 0; k < shots; ++k) out[k] = 0;
+        return QVNT_OK;
+    }
+    for (size_t k = 0; k < shots; ++k)
+        if (!(u[k] >= 0.0 && u[k] < 1.0)) return fail(QVNT_E_ARG, "draws must lie in [0, 1)");
+    // workspace layout: [tree levels | total | draws | outcomes], 256-byte aligned
+    const uint64_t count = 1ull << r->q_num;
+    const size_t off_total = align256(tree_doubles(count) * sizeof(double));
+    const size_t off_u = off_total + 256;
+    const size_t off_out = off_u + align256(shots * sizeof(double));
+    const size_t need = off_out + align256(shots * sizeof(uint64_t));
+    if (need > kScratchBytes)
+        return fail(QVNT_E_ARG, "allocation-free shots need " + std::to_string(need) + " bytes of workspace; the "
+                                "fixed workspace holds " + std::to_string(kScratchBytes) +
+                                " (registers up to 30 qubits, fewer shots at 30)");
+    unsigned char* ws = nullptr;
+    if (int rc = scratch_address(r->device, &ws)) return rc;
+    double* tree = reinterpret_cast<double*>(ws);
+    double* total_dev = reinterpret_cast<double*>(ws + off_total);
+    double* u_dev = reinterpret_cast<double*>(ws + off_u);
+    uint64_t* out_dev = reinterpret_cast<uint64_t*>(ws + off_out);
+    QVNT_HIP(hipMemcpyAsync(u_dev, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
+    if (int rc = sample_into(r, tree, count, u_dev, shots, out_dev, total_dev, false)) return rc;
+    double total = 0.0;
+    QVNT_HIP(hipMemcpyAsync(out, out_dev, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
+    QVNT_HIP(hipMemcpyAsync(&total, total_dev, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    if (!(total > 0.0) || !std::isfinite(total)) return fail(QVNT_E_ZERO_NORM, "measurement of a zero-norm or non-finite state");
+    for (size_t k = 0; k < shots; ++k) out[k] &= mask;
+    return QVNT_OK;
+}

@@ -9,8 +9,9 @@
 
 namespace qvnt {
 
+// reg may be null: no timing (the allocation-free path creates no events).
 Timed::Timed(qvnt_reg* reg, int kc, double by) : r(reg), kclass(kc), bytes(by) {
-    if (!r->timing) return;
+    if (!r || !r->timing) return;
     for (hipEvent_t* e : {&a, &b}) {
         if (!r->event_pool.empty()) {
             *e = r->event_pool.back();
@@ -23,7 +24,7 @@ Timed::Timed(qvnt_reg* reg, int kc, double by) : r(reg), kclass(kc), bytes(by) {
 }
 
 Timed::~Timed() {
-    if (!r->timing || !a || !b) return;
+    if (!r || !r->timing || !a || !b) return;
     (void)hipEventRecord(b, r->stream);
     r->pending.push_back({kclass, a, b, bytes});
 }

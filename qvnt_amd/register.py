@@ -242,6 +242,15 @@ class QReg:
                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return out
 
+    def sample_mask_noalloc(self, mask, u):
+        """sample_mask through the allocation-free entry point (a fixed
+        device workspace shared by every call on the GPU)."""
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.empty(len(u), np.uint64)
+        check(lib.qvnt_reg_sample_mask_noalloc(self._h, mask, u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(u),
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
+
     def compare(self, other):
         """(max |a_i - b_i|, infidelity, <a|b>, |a|^2, |b|^2), computed on the device in FP64."""
         out = (ctypes.c_double * 5)()
