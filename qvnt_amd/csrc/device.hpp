@@ -71,22 +71,31 @@ __device__ __forceinline__ uint64_t deposit(uint64_t g, const Deposit& d) {
     return g | d.set;
 }
 
-// Gate parameters in the amplitude precision R.
-template <class R>
+// Gate parameters, FP64 for either amplitude precision: the stored phases
+// (the QFT ladder's controlled-RZ angles among them) and matrices keep the
+// reference's double values, and the formulas below compute in FP64.
 struct Par {
     uint64_t a, b;       // a: act mask (orbit mask for pair kinds); b: H2/U2 b_mask
-    R re, im;            // stored phase
-    R m[32];             // U1 / U2 matrices
+    double re, im;       // stored phase
+    double m[32];        // U1 / U2 matrices
     uint32_t dagger;
     uint32_t i_pow;      // Y: !(popcount(a) + 1) as usize (atomic/y.rs:11)
 };
 
 constexpr double kFrac1Sqrt2 = 0.70710678118654752440;
 
+// Amplitudes enter a formula in FP64 and leave it rounded once to the
+// register's precision R (for R = double both conversions are the identity,
+// so FP64 registers round exactly like the reference).
+template <class R>
+__device__ __forceinline__ c64 up(cplx<R> p) { return c64{(double)p.re, (double)p.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> down(c64 p) { return cplx<R>{(R)p.re, (R)p.im}; }
+
 // ------------------------------------------------------------- diagonals
 // out[idx] = f(psi[idx]); kinds Z, S, T, RZ, RZZ, PHI.
-template <int K, class R>
-__device__ __forceinline__ cplx<R> f_diag(const Par<R>& g, uint64_t idx, cplx<R> p) {
+template <int K>
+__device__ __forceinline__ c64 f_diag64(const Par& g, uint64_t idx, c64 p) {
     if constexpr (K == QVNT_Z) {                                   // z.rs:15-21
         return (dpopc(idx & g.a) & 1u) ? cneg(p) : p;
     } else if constexpr (K == QVNT_S) {                            // s.rs:19-25
@@ -96,28 +105,32 @@ __device__ __forceinline__ cplx<R> f_diag(const Par<R>& g, uint64_t idx, cplx<R>
     } else if constexpr (K == QVNT_T) {                            // t.rs:24-35
         uint64_t count = dpopc(idx & g.a);
         if (g.dagger) count = (~count) + 1u;
-        const cplx<R> z = rotate(p, count >> 1);
-        const cplx<R> e{(R)kFrac1Sqrt2, (R)kFrac1Sqrt2};
+        const c64 z = rotate(p, count >> 1);
+        const c64 e{kFrac1Sqrt2, kFrac1Sqrt2};
         return (count & 1u) ? cmul(e, z) : z;
     } else if constexpr (K == QVNT_RZ) {                           // rz.rs:19-25
-        cplx<R> ph{g.re, g.im};
+        c64 ph{g.re, g.im};
         if ((idx & g.a) == 0) ph.im = -ph.im;
         return cmul(ph, p);
     } else if constexpr (K == QVNT_RZZ) {                          // rzz.rs:19-25
-        cplx<R> ph{g.re, g.im};
+        c64 ph{g.re, g.im};
         if ((dpopc(idx & g.a) & 1u) == 0) ph.im = -ph.im;
         return cmul(ph, p);
     } else {                                                       // PHI: diag(1, e^{i th})
-        const cplx<R> ph{g.re, g.im};
+        const c64 ph{g.re, g.im};
         return (idx & g.a) ? cmul(ph, p) : p;
     }
+}
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_diag(const Par& g, uint64_t idx, cplx<R> p) {
+    return down<R>(f_diag64<K>(g, idx, up(p)));
 }
 
 // ---------------------------------------------------------- pair orbits
 // out[idx] = f(psi[idx], psi[idx ^ m]); `self` = psi[idx],
 // `other` = psi[idx ^ m] with m the orbit mask.
-template <int K, class R>
-__device__ __forceinline__ cplx<R> f_pair(const Par<R>& g, uint64_t idx, cplx<R> self, cplx<R> other) {
+template <int K>
+__device__ __forceinline__ c64 f_pair64(const Par& g, uint64_t idx, c64 self, c64 other) {
     if constexpr (K == QVNT_X) {                                   // x.rs:15-17
         return other;
     } else if constexpr (K == QVNT_Y) {                            // y.rs:17-23
@@ -127,55 +140,64 @@ __device__ __forceinline__ cplx<R> f_pair(const Par<R>& g, uint64_t idx, cplx<R>
     } else if constexpr (K == QVNT_RX || K == QVNT_RXX) {          // rx.rs:18-24, rxx.rs:19-25
         return {self.re * g.re + other.im * g.im, self.im * g.re - other.re * g.im};
     } else if constexpr (K == QVNT_RY) {                           // ry.rs:19-29
-        R s = g.im;
+        double s = g.im;
         if ((idx & g.a) == 0) s = -s;
         return {self.re * g.re + other.re * s, self.im * g.re + other.im * s};
     } else if constexpr (K == QVNT_RYY) {                          // ryy.rs:19-29
-        R s = g.im;
+        double s = g.im;
         if ((dpopc(idx & g.a) & 1u) == 0) s = -s;
         return {self.re * g.re + other.im * s, self.im * g.re - other.re * s};
     } else if constexpr (K == QVNT_H1) {                           // h1.rs:16-22
-        cplx<R> p0 = self;
+        c64 p0 = self;
         if (idx & g.a) p0 = cneg(p0);
-        return cscale(cadd(p0, other), (R)kFrac1Sqrt2);
+        return cscale(cadd(p0, other), kFrac1Sqrt2);
     } else if constexpr (K == QVNT_U1) {                           // u1.rs:17-25
         if ((idx & g.a) == 0)
-            return cadd(cmul(cplx<R>{g.m[0], g.m[1]}, self), cmul(cplx<R>{g.m[2], g.m[3]}, other));
-        return cadd(cmul(cplx<R>{g.m[4], g.m[5]}, other), cmul(cplx<R>{g.m[6], g.m[7]}, self));
+            return cadd(cmul(c64{g.m[0], g.m[1]}, self), cmul(c64{g.m[2], g.m[3]}, other));
+        return cadd(cmul(c64{g.m[4], g.m[5]}, other), cmul(c64{g.m[6], g.m[7]}, self));
     } else if constexpr (K == QVNT_SWAP) {                         // swap.rs:16-22 (odd orbits)
         return other;
     } else if constexpr (K == QVNT_ISWAP) {                        // i_swap.rs:20-37 (odd orbits)
         if (g.dagger) return {other.im, -other.re};
         return {-other.im, other.re};
     } else if constexpr (K == QVNT_SQRT_SWAP) {                    // sqrt_swap.rs:20-37 (odd orbits)
-        const cplx<R> p0 = self, p1 = other;
+        const c64 p0 = self, p1 = other;
         if (g.dagger)
-            return {(R)0.5 * (p0.re + p0.im + p1.re - p1.im), (R)0.5 * (p0.im - p0.re + p1.im + p1.re)};
-        return {(R)0.5 * (p0.re - p0.im + p1.re + p1.im), (R)0.5 * (p0.im + p0.re + p1.im - p1.re)};
+            return {0.5 * (p0.re + p0.im + p1.re - p1.im), 0.5 * (p0.im - p0.re + p1.im + p1.re)};
+        return {0.5 * (p0.re - p0.im + p1.re + p1.im), 0.5 * (p0.im + p0.re + p1.im - p1.re)};
     } else {                                                       // sqrt_i_swap.rs:20-37 (odd orbits)
-        const cplx<R> p0 = self, p1 = other;
-        if (g.dagger) return {(R)kFrac1Sqrt2 * (p0.re + p1.im), (R)kFrac1Sqrt2 * (p0.im - p1.re)};
-        return {(R)kFrac1Sqrt2 * (p0.re - p1.im), (R)kFrac1Sqrt2 * (p0.im + p1.re)};
+        const c64 p0 = self, p1 = other;
+        if (g.dagger) return {kFrac1Sqrt2 * (p0.re + p1.im), kFrac1Sqrt2 * (p0.im - p1.re)};
+        return {kFrac1Sqrt2 * (p0.re - p1.im), kFrac1Sqrt2 * (p0.im + p1.re)};
     }
+}
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_pair(const Par& g, uint64_t idx, cplx<R> self, cplx<R> other) {
+    return down<R>(f_pair64<K>(g, idx, up(self), up(other)));
 }
 
 // ---------------------------------------------------------- quad orbits
 // v[k] = psi[base | sel_k] with sel bits (b, a) = k; output for v-slot `s`.
-template <int K, class R>
-__device__ __forceinline__ cplx<R> f_quad(const Par<R>& g, int s, const cplx<R>* v) {
+template <int K>
+__device__ __forceinline__ c64 f_quad64(const Par& g, int s, const c64* v) {
     if constexpr (K == QVNT_H2) {                                  // h2.rs:22-38
-        cplx<R> t0 = v[s], t1 = v[s ^ 1], t2 = v[s ^ 2], t3 = v[s ^ 3];
+        c64 t0 = v[s], t1 = v[s ^ 1], t2 = v[s ^ 2], t3 = v[s ^ 3];
         if (s & 1) { t0 = cneg(t0); t2 = cneg(t2); }
         if (s & 2) { t0 = cneg(t0); t1 = cneg(t1); }
-        return cscale(cadd(cadd(cadd(t0, t1), t2), t3), (R)0.5);
+        return cscale(cadd(cadd(cadd(t0, t1), t2), t3), 0.5);
     } else {                                                       // u2.rs:22-49
-        const R* m = g.m + 8 * s;
-        cplx<R> acc = cmul(cplx<R>{m[0], m[1]}, v[0]);
-        acc = cadd(acc, cmul(cplx<R>{m[2], m[3]}, v[1]));
-        acc = cadd(acc, cmul(cplx<R>{m[4], m[5]}, v[2]));
-        acc = cadd(acc, cmul(cplx<R>{m[6], m[7]}, v[3]));
+        const double* m = g.m + 8 * s;
+        c64 acc = cmul(c64{m[0], m[1]}, v[0]);
+        acc = cadd(acc, cmul(c64{m[2], m[3]}, v[1]));
+        acc = cadd(acc, cmul(c64{m[4], m[5]}, v[2]));
+        acc = cadd(acc, cmul(c64{m[6], m[7]}, v[3]));
         return acc;
     }
+}
+template <int K, class R>
+__device__ __forceinline__ cplx<R> f_quad(const Par& g, int s, const cplx<R>* v) {
+    const c64 w[4] = {up(v[0]), up(v[1]), up(v[2]), up(v[3])};
+    return down<R>(f_quad64<K>(g, s, w));
 }
 
 }  // namespace qvnt

@@ -52,7 +52,7 @@ __device__ __forceinline__ void st(cplx<R>* p, cplx<R> v) {
 }
 
 template <int K, class R, int U, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+__global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
   const uint64_t B = blockDim.x, per = B * U;
   const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, u
 }
 
 template <int K, class R, int U, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+__global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
   const uint64_t B = blockDim.x, per = B * U;
   const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, u
 }
 
 template <int K, class R, int U, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par<R> g) {
+__global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
   const uint64_t B = blockDim.x, per = B * U;
   const uint64_t nchunks = (groups + per - 1) / per;
   for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, u
 // every wave is either whole or idle and all of its lanes shuffle.
 template <int K, class R, int U>
 __global__ __launch_bounds__(kMaxBlock) void k_pair_lanes(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep,
-                                                          Par<R> g) {
+                                                          Par g) {
     const uint64_t B = blockDim.x, per = B * U;
     const uint64_t nchunks = (groups + per - 1) / per;
     const int lm = (int)g.a;
@@ -272,14 +272,14 @@ void fix_bits(Deposit& d, uint64_t bits, bool one) {
     if (one) d.set |= bits;
 }
 
-template <class R>
-Par<R> make_par(const qvnt_gate& g) {
-    Par<R> p{};
+// Gate parameters in FP64 for both amplitude precisions.
+Par make_par(const qvnt_gate& g) {
+    Par p{};
     p.a = g.a;
     p.b = g.b;
-    p.re = (R)g.re;
-    p.im = (R)g.im;
-    for (int k = 0; k < 32; ++k) p.m[k] = (R)g.m[k];
+    p.re = g.re;
+    p.im = g.im;
+    for (int k = 0; k < 32; ++k) p.m[k] = g.m[k];
     p.dagger = g.dagger ? 1u : 0u;
     p.i_pow = (~(uint32_t)(popc(g.a) + 1u)) & 3u;  // atomic/y.rs:11
     return p;
@@ -290,7 +290,7 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
     Deposit d = d0;
     const uint32_t S = U == 2 ? skew_setting(cls, sizeof(cplx<R>), r->len * sizeof(cplx<R>)) : 0u;
     d.skew = (S && groups >= (2ull << S)) ? S : 0u;
-    const Par<R> p = make_par<R>(gate);
+    const Par p = make_par(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
     const unsigned B = block_setting();
     const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
@@ -348,7 +348,7 @@ int launch_lanes(qvnt_reg* r, const qvnt_gate& gate) {
     Deposit d{};
     fix_bits(d, gate.ctrl, true);
     const uint64_t groups = r->len >> d.n;
-    const Par<R> p = make_par<R>(gate);
+    const Par p = make_par(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
     const unsigned B = block_setting();
     constexpr int U = 4;  // four amplitudes per lane: as many loads in flight as k_pair's two orbits

@@ -322,3 +322,85 @@ def test_compare_two_registers():
     want = 1.0 - abs(np.vdot(sa, sb)) ** 2 / (np.vdot(sa, sa).real * np.vdot(sb, sb).real)
     assert abs(infid - want) <= 1e-12
     assert 0.0 <= infid < 1e-5
+
+
+def test_fp32_gates_round_once_from_fp64(orc):
+    """FP32 registers keep every gate parameter (the QFT ladder's phases
+    among them) in FP64 and round only the output amplitude: after each
+    SingleOp the FP32 state equals the FP64 oracle's result on the
+    promoted FP32 input, rounded to FP32, bit for bit -- every kind, with
+    controls and daggers."""
+    g = np.random.default_rng(4242)
+    n = 10
+    psi = (g.normal(size=1 << n) + 1j * g.normal(size=1 << n)).astype(np.complex64)
+    specs = random_circuit(g, n, 40) + [("qft", ((1 << n) - 1,), 0, False)]
+    gates = build(specs, orc.op).gates
+    dev = build(specs, op).gates()
+    assert len(gates) == len(dev)
+    ref = orc.Reg(n)
+    with QReg(n, prec="f32") as r:
+        r.set_state(psi)
+        for k, (gc, gd) in enumerate(zip(gates, dev)):
+            ref.set_state(psi.astype(np.complex128))
+            ref.apply(orc.Op([gc]))
+            want = ref.state().astype(np.complex64)
+            r.apply(qvnt_amd.from_gates([gd]))
+            got = r.get_state()
+            assert np.array_equal(got, want), (k, qvnt_amd.KINDS[gd.kind], float(np.max(np.abs(got - want))))
+            psi = got
+
+
+def test_fp32_qft20_against_fp64_oracle(orc):
+    """qft on all 20 qubits of an FP32 register (210 SingleOps, 190 of them
+    controlled-RZ phases) against the FP64 oracle: infidelity <= 1e-5
+    (reported: far below)."""
+    n, state = 20, 0x5A5A5
+    ref = orc.Reg(n, state, threads=8)
+    ref.apply(orc.op.qft((1 << n) - 1))
+    want = ref.state()
+    with QReg.with_state(n, state, prec="f32") as r:
+        r.apply(op.qft((1 << n) - 1))
+        got = r.get_state().astype(np.complex128)
+    err = float(np.max(np.abs(got - want)))
+    infid = 1.0 - abs(np.vdot(want, got)) ** 2 / (np.vdot(got, got).real * np.vdot(want, want).real)
+    say("fp32 qft(20): max |d| %.2e, infidelity %.2e" % (err, infid))
+    assert err <= 1e-5 and infid <= 1e-5
+
+
+@pytest.mark.parametrize("form", ["fused", "plain"])
+def test_tfim_ring_28q_fp32_against_fp64(form):
+    """The 28-qubit TFIM ring (50 Trotter steps) in an FP64 and an FP32
+    register side by side: after every step the device comparison gives
+    max |dpsi| <= 1e-5 and infidelity <= 1e-5; both norms stay 1."""
+    n = 28
+    say("tfim pair 28 q, %s" % form)
+    step = circuits.build(circuits.tfim_step(n), op)
+    worst = (0.0, 0.0)
+    with QReg(n) as a, QReg(n, prec="f32") as b:
+        for k in range(50):
+            if form == "fused":
+                a.apply_fused(step)
+                b.apply_fused(step)
+            else:
+                a.apply(step)
+                b.apply(step)
+            err, infid, _, na, nb = a.compare(b)
+            assert err <= 1e-5 and infid <= 1e-5, (k, err, infid)
+            assert abs(na - 1.0) <= 1e-12 and abs(nb - 1.0) <= 1e-5
+            worst = (max(worst[0], err), max(worst[1], infid))
+    say("  worst step: max |dpsi| %.2e, infidelity %.2e" % worst)
+
+
+def test_u2_controlled_on_its_b_qubit_is_refused_by_apply():
+    """op::matrix2(a, b, m).c(b) is a valid operator (acts_on = a only,
+    atomic/u2.rs:70-72) that the engine refuses to apply, leaving the state
+    untouched."""
+    from qvnt_amd._lib import E_CTRL_OVERLAP
+    m = np.eye(4, dtype=np.complex128)[[1, 0, 3, 2]]
+    cu = op.matrix2(0b001, 0b010, m).c(0b010)
+    with QReg.with_state(3, 0b011) as r:
+        before = r.get_state()
+        with pytest.raises(qvnt_amd.QvntError) as e:
+            r.apply(cu)
+        assert e.value.code == E_CTRL_OVERLAP
+        assert np.array_equal(r.get_state(), before)
