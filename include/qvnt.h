@@ -241,6 +241,12 @@ int qvnt_reg_compare(qvnt_reg* a, qvnt_reg* b, double out[5]);
  * sizeof(double or float)).  Blocking. */
 int qvnt_reg_get_state(qvnt_reg* reg, void* dst, size_t bytes);
 int qvnt_reg_set_state(qvnt_reg* reg, const void* src, size_t bytes);
+/* Debug of QReg (quant.rs:603-623): "QReg { 0: Complex { re: 0.25, im: 0.0 },
+ * ..., 7: ..., .. }" -- the first min(2^n, 8) amplitudes with Rust's f64
+ * Debug formatting, ".." when the register holds more (FP32 amplitudes are
+ * printed widened to f64).  Writes at most cap bytes (NUL-terminated);
+ * *need gets the full length + 1.  Synchronises. */
+int qvnt_reg_debug(qvnt_reg* reg, char* buf, size_t cap, size_t* need);
 /* Wait for all work queued on the register's stream. */
 int qvnt_reg_sync(qvnt_reg* reg);
 

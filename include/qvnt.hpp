@@ -6,14 +6,18 @@
 //            Option), `.dgr()`, `.act_on()`, Debug name    src/operator/multi/mod.rs
 //   QReg::{new_, with_state, apply, apply_fused, get_probabilities,
 //          get_absolute, measure_mask, measure}            src/register/quant.rs
-//   CReg (measurement result)                               src/register/class.rs
+//   CReg (classical register, measurement result)           src/register/class.rs
+//   VReg (mask building by position / predicate)            src/register/virtl.rs
 // Where the reference panics (a rotation mask with the wrong bit count, a
 // measurement of a zero state) the facade throws qvnt::Error.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
+#include <initializer_list>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -118,11 +122,95 @@ inline MultiOp qft(uint64_t m) { return detail::make(qvnt_op_qft, m); }
 inline MultiOp qft_swapped(uint64_t m) { return detail::make(qvnt_op_qft_swapped, m); }
 }  // namespace op
 
-// CReg: the measured value, bits in place (class.rs:66-74, quant.rs:500).
-struct CReg {
-    uint32_t q_num = 0;
-    uint64_t value = 0;
-    uint64_t get() const { return value; }
+// CReg: a classical register of q_num bits (src/register/class.rs:50-155);
+// measure_mask returns one with the measured bits in place (quant.rs:500).
+class CReg {
+public:
+    CReg() = default;
+    // class.rs:60-62, 66-74
+    static CReg new_(uint32_t q_num) { return with_state(q_num, 0); }
+    static CReg with_state(uint32_t q_num, uint64_t state) {
+        CReg c;
+        c.value_ = state;
+        c.set_num(q_num);
+        return c;
+    }
+    uint32_t num() const { return q_num_; }                         // class.rs:76-78
+    void set_num(uint32_t q_num) {                                  // class.rs:80-83
+        q_num_ = q_num;
+        q_mask_ = q_num >= 64 ? ~0ull : (1ull << q_num) - 1u;       // wrapping_shl(q).wrapping_sub(1)
+    }
+    void reset(uint64_t i_state) { value_ = i_state & q_mask_; }    // class.rs:85-87
+    void set(bool bit, uint64_t mask) { value_ = bit ? (value_ | mask) : (value_ & ~mask); }  // class.rs:89-95
+    void xor_(bool bit, uint64_t mask) {                            // class.rs:97-101 (`xor` is a C++ keyword)
+        if (bit) value_ ^= mask;
+    }
+    uint64_t get() const { return value_; }                         // class.rs:111-113
+    // the bits of value at the set bits of mask & q_mask, packed from bit 0 (class.rs:115-125)
+    uint64_t get_by_mask(uint64_t mask) const {
+        uint64_t out = 0;
+        unsigned k = 0;
+        for (uint64_t m = mask & q_mask_; m; m &= m - 1, ++k)
+            if (value_ & (m & (0 - m))) out |= 1ull << k;
+        return out;
+    }
+    // tensor product, self on the low bits (class.rs:103-109, 142-154)
+    CReg operator*(const CReg& o) const { return with_state(q_num_ + o.q_num_, value_ | (o.value_ << q_num_)); }
+    CReg& operator*=(const CReg& o) { return *this = *this * o; }
+    bool operator==(const CReg& o) const { return value_ == o.value_ && q_num_ == o.q_num_ && q_mask_ == o.q_mask_; }
+    // Debug: "(b_{n-1} .. b_0)" (class.rs:128-140)
+    std::string debug() const {
+        std::string s;
+        for (uint64_t m = q_mask_; m; m &= m - 1) s.insert(s.begin(), (value_ & (m & (0 - m))) ? '1' : '0');
+        return "(" + s + ")";
+    }
+
+private:
+    uint64_t value_ = 0;
+    uint32_t q_num_ = 0;
+    uint64_t q_mask_ = 0;
+};
+
+// VReg: single-bit masks of a register's qubits, indexed to build masks
+// (src/register/virtl.rs:85-157).
+class VReg {
+public:
+    static VReg new_(uint32_t num) { return from(num >= 64 ? ~0ull : (1ull << num) - 1u); }  // virtl.rs:88-91
+    static VReg from(uint64_t mask) {                               // virtl.rs:93-104
+        VReg v;
+        for (uint64_t m = mask; m; m &= m - 1) v.bits_.push_back(m & (0 - m));
+        return v;
+    }
+    size_t len() const { return bits_.size(); }
+    uint64_t operator[](size_t idx) const { return bits_.at(idx); }  // virtl.rs:112-119
+    // the OR of the masks at the positions where f holds (virtl.rs:121-137)
+    template <class F, class = decltype(std::declval<F>()(size_t(0)))>
+    uint64_t operator[](F f) const {
+        uint64_t out = 0;
+        for (size_t i = 0; i < bits_.size(); ++i)
+            if (f(i)) out |= bits_[i];
+        return out;
+    }
+    // positions listed (virtl.rs:139-145)
+    uint64_t operator[](std::initializer_list<size_t> pos) const {
+        return (*this)[([&](size_t i) { return std::find(pos.begin(), pos.end(), i) != pos.end(); })];
+    }
+    uint64_t all() const {                                          // v[..] (virtl.rs:147-157)
+        return (*this)[([](size_t) { return true; })];
+    }
+    // Debug: the masks as zero-padded 32-digit hex (virtl.rs:106-110)
+    std::string debug() const {
+        std::string s = "[";
+        char buf[40];
+        for (size_t i = 0; i < bits_.size(); ++i) {
+            std::snprintf(buf, sizeof buf, "%032llx", (unsigned long long)bits_[i]);
+            s += (i ? ", " : "") + std::string(buf);
+        }
+        return s + "]";
+    }
+
+private:
+    std::vector<uint64_t> bits_;
 };
 
 class QReg {
@@ -163,7 +251,7 @@ public:
         uint64_t out = 0;
         const double draw = u ? *u : qvnt_uniform(seed_, 0, draws_++);
         check(qvnt_reg_measure_mask(h_, mask, draw, renorm ? QVNT_MEASURE_RENORM : 0u, &out));
-        return CReg{q_num_, out};
+        return CReg::with_state(q_num_, out);
     }
     CReg measure(std::optional<double> u = std::nullopt) { return measure_mask(~0ull, u); }
     std::vector<C> get_state() {
@@ -217,6 +305,22 @@ public:
         std::vector<uint64_t> hist(len());
         check(qvnt_reg_sample_all(h_, count, seed_, stream, hist.data(), hist.size()));
         return hist;
+    }
+    // QReg::get_vreg / get_vreg_by (quant.rs:232-243): nullopt for bits beyond the register
+    VReg get_vreg() const { return VReg::new_(q_num_); }
+    std::optional<VReg> get_vreg_by(uint64_t mask) const {
+        const uint64_t q_mask = q_num_ >= 64 ? ~0ull : (1ull << q_num_) - 1u;
+        if (mask & ~q_mask) return std::nullopt;
+        return VReg::from(mask);
+    }
+    // Debug of QReg (quant.rs:603-623): "QReg { 0: Complex { re: .., im: .. }, .. }"
+    std::string debug() {
+        size_t need = 0;
+        check(qvnt_reg_debug(h_, nullptr, 0, &need));
+        std::string s(need, '\0');
+        check(qvnt_reg_debug(h_, s.data(), s.size(), &need));
+        s.resize(need - 1);
+        return s;
     }
     qvnt_reg* handle() { return h_; }
 

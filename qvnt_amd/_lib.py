@@ -98,6 +98,7 @@ def _declare(L):
         "qvnt_reg_get_state": ([vp, vp, sz], i),
         "qvnt_reg_set_state": ([vp, vp, sz], i),
         "qvnt_reg_sync": ([vp], i),
+        "qvnt_reg_debug": ([vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)], i),
         "qvnt_reg_set_timing": ([vp, i], i),
         "qvnt_reg_set_dense": ([vp, u32], i),
         "qvnt_reg_kernel_stats": ([vp, ctypes.POINTER(KStats), i], i),

@@ -37,5 +37,7 @@ inline uint64_t touched_bits(const qvnt_gate& g) {
 bool is_diagonal_kind(uint32_t kind);
 // the reference's unitarity rule for a 2x2 complex matrix (interleaved re, im)
 bool is_unitary_m1(const double* m);
+// num-complex's Debug of a Complex<f64> ("Complex { re: 0.25, im: 0.0 }")
+std::string complex_debug_value(double re, double im);
 
 }  // namespace qvnt

@@ -302,6 +302,12 @@ std::string gate_name(const qvnt_gate& g) {
 
 }  // namespace
 
+// num-complex's Debug of a Complex<f64>: "Complex { re: .., im: .. }".
+std::string complex_debug_value(double re, double im) {
+    const double m[2] = {re, im};
+    return complex_debug(m, 0);
+}
+
 bool is_unitary_m1(const double* m) { return unitary_1(m); }
 
 }  // namespace qvnt

@@ -1,9 +1,11 @@
 // C ABI of the device register: QReg::new / with_state / apply /
 // get_absolute / get_probabilities / measure_mask (src/register/quant.rs),
 // plus state copy-in/out and per-kernel-class timing.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
 
 #include "reg.hpp"
 
@@ -263,6 +265,37 @@ int qvnt_reg_set_state(qvnt_reg* r, const void* src, size_t bytes) {
     if (!src || bytes != r->len * r->amp_bytes()) return fail(QVNT_E_ARG, "state copy needs len * amplitude bytes");
     QVNT_HIP(hipMemcpyAsync(r->psi, src, bytes, hipMemcpyHostToDevice, r->stream));
     QVNT_HIP(hipStreamSynchronize(r->stream));
+    return QVNT_OK;
+}
+
+int qvnt_reg_debug(qvnt_reg* r, char* buf, size_t cap, size_t* need) {
+    if (int rc = check_reg(r)) return rc;
+    // Debug of QReg (quant.rs:603-623): the first min(2^n, 8) amplitudes
+    // (MAX_LEN_TO_DISPLAY, quant.rs:16) as struct fields, ".." after them
+    // when the register holds more
+    constexpr uint64_t kMaxShown = 8;
+    const uint64_t n = 1ull << r->q_num;
+    const uint64_t shown = n < kMaxShown ? n : kMaxShown;
+    double amps[2 * kMaxShown];
+    if (r->prec == QVNT_F64) {
+        QVNT_HIP(hipMemcpyAsync(amps, r->psi, shown * 16, hipMemcpyDeviceToHost, r->stream));
+        QVNT_HIP(hipStreamSynchronize(r->stream));
+    } else {
+        float f[2 * kMaxShown];
+        QVNT_HIP(hipMemcpyAsync(f, r->psi, shown * 8, hipMemcpyDeviceToHost, r->stream));
+        QVNT_HIP(hipStreamSynchronize(r->stream));
+        for (uint64_t k = 0; k < 2 * shown; ++k) amps[k] = (double)f[k];
+    }
+    std::string s = "QReg { ";
+    for (uint64_t k = 0; k < shown; ++k)
+        s += (k ? ", " : "") + std::to_string(k) + ": " + complex_debug_value(amps[2 * k], amps[2 * k + 1]);
+    s += n > kMaxShown ? ", .. }" : " }";
+    if (need) *need = s.size() + 1;
+    if (buf && cap) {
+        const size_t m = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), m);
+        buf[m] = '\0';
+    }
     return QVNT_OK;
 }
 

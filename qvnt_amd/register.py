@@ -274,6 +274,17 @@ class QReg:
     def sync(self):
         check(lib.qvnt_reg_sync(self._h))
 
+    def debug(self):
+        """Debug of QReg (quant.rs:603-623): the first min(2^n, 8) amplitudes."""
+        need = ctypes.c_size_t(0)
+        check(lib.qvnt_reg_debug(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        check(lib.qvnt_reg_debug(self._h, buf, need.value, ctypes.byref(need)))
+        return buf.value.decode()
+
+    def __repr__(self):
+        return self.debug() if self._h else "QReg(closed)"
+
     # -------------------------------------------- reset, algebra, readback
     @classmethod
     def _adopt(cls, h, q_num, prec, seed=DEFAULT_SEED):

@@ -40,6 +40,63 @@ static void rotation_mask_panics() {
     try { op::rx(1.0, 0b11); } catch (const Error& e) { threw = e.code == QVNT_E_INVALID_MASK; }
     EXPECT(threw);
 }
+// src/register/class.rs:14-31 (doctests: with_state, tensor product) and
+// :37-48 (get, Debug); class.rs:159-165 (display)
+static void creg_doc() {
+    CReg c0 = CReg::with_state(4, 11);
+    CReg c1 = CReg::with_state(4, 7);
+    CReg c = c0 * c1;
+    EXPECT(c == CReg::with_state(8, 123));
+    EXPECT(c.get() == 123);
+    EXPECT(c.debug() == "(01111011)");
+    EXPECT(CReg::with_state(17, 123).debug() == "(00000000001111011)");
+    CReg d = CReg::with_state(4, 11);
+    d *= CReg::with_state(4, 7);
+    EXPECT(d == c);
+}
+// src/register/class.rs:85-125: reset, set, xor, get_by_mask
+static void creg_bits() {
+    CReg c = CReg::new_(6);
+    c.reset(0b1101011);                    // masked to 6 bits
+    EXPECT(c.get() == 0b101011);
+    c.set(true, 0b010100);
+    EXPECT(c.get() == 0b111111);
+    c.set(false, 0b000011);
+    EXPECT(c.get() == 0b111100);
+    c.xor_(true, 0b100001);
+    EXPECT(c.get() == 0b011101);
+    c.xor_(false, 0b111111);               // bit false: unchanged
+    EXPECT(c.get() == 0b011101);
+    // bits 0, 2, 3 of 0b011101 are 1, 1, 1; bit 5 (mask beyond 6 bits drops bit 7) is 0
+    EXPECT(c.get_by_mask(0b10101101) == 0b0111);
+    EXPECT(c.get_by_mask(0) == 0);
+    c.set_num(3);
+    EXPECT(c.num() == 3 && c.get() == 0b011101);   // set_num keeps the value
+    EXPECT(c.get_by_mask(0b111) == 0b101);
+}
+// src/register/virtl.rs:161-178 (index)
+static void vreg_index() {
+    auto even = [](size_t i) { return i % 2 == 0; };
+    auto odd = [](size_t i) { return i % 2 != 0; };
+    VReg v = VReg::new_(8);
+    EXPECT(v[3] == 0b00001000);
+    EXPECT(v[even] == 0b01010101);
+    EXPECT(v[odd] == 0b10101010);
+    EXPECT(v.all() == 0b11111111);
+    EXPECT(v[([](size_t) { return true; })] == 0b11111111);
+    EXPECT((v[{0, 7}]) == 0b10000001);
+}
+// src/register/virtl.rs:14-84 (doctests)
+static void vreg_doc() {
+    VReg v = VReg::new_(8);
+    EXPECT((v[{3, 4, 5}]) == 0b00111000);
+    EXPECT(v[([](size_t i) { return i % 2 != 0; })] == 0b10101010);
+    VReg e = VReg::from(v[([](size_t i) { return i % 2 == 0; })]);
+    VReg o = VReg::from(v[([](size_t i) { return i % 2 != 0; })]);
+    EXPECT(e.all() == 0b01010101 && o.all() == 0b10101010);
+    EXPECT(e.len() == 4 && e[1] == 0b100);
+    EXPECT(VReg::from(0b101).debug() == "[00000000000000000000000000000001, 00000000000000000000000000000004]");
+}
 // src/register/quant.rs:642-677 (quantum_reg)
 static void quantum_reg() {
     QReg reg = QReg::with_state(4, 0b1100);
@@ -50,6 +107,10 @@ static void quantum_reg() {
     const double want[16] = {0.25, 0.25, 0.25, 0.0, -0.25, -0.25, -0.25, 0.0, -0.5, 0.0, 0.25, 0.0, 0.5, 0.0, -0.25, 0.0};
     auto psi = reg.get_state();
     for (int i = 0; i < 16; ++i) EXPECT(psi[i] == std::complex<double>(want[i], 0.0));
+    EXPECT(reg.debug() ==
+           "QReg { 0: Complex { re: 0.25, im: 0.0 }, 1: Complex { re: 0.25, im: 0.0 }, 2: Complex { re: 0.25, im: 0.0 }, "
+           "3: Complex { re: 0.0, im: 0.0 }, 4: Complex { re: -0.25, im: 0.0 }, 5: Complex { re: -0.25, im: 0.0 }, "
+           "6: Complex { re: -0.25, im: 0.0 }, 7: Complex { re: 0.0, im: 0.0 }, .. }");
     EXPECT((reg.measure_mask(mask).get() & ~mask) == 0);
 }
 // src/register/quant.rs:81-96 (doctest: Bell state)
@@ -90,12 +151,27 @@ static void histogram() {
         EXPECT(hist.size() == 256 && sum == 2048);
     }
 }
-// src/register/virtl.rs (virtual_regs): reg = new(0) * new(3) * new(2)
+// src/register/virtl.rs:180-218 (virtual_regs): reg = new(0) * new(3) * new(2)
 static void virtual_regs_sizes() {
     QReg a = QReg::new_(0), b = QReg::new_(3), c = QReg::new_(2);
     QReg ab = a * b;
     QReg reg = ab * c;
     EXPECT(reg.num() == 5 && reg.get_probabilities()[0] == 1.0);
+    VReg r = reg.get_vreg();
+    std::optional<VReg> m = reg.get_vreg_by(0b01101), x = reg.get_vreg_by(0b00111), y = reg.get_vreg_by(0b11000);
+    EXPECT(m && x && y && !reg.get_vreg_by(0b100000));
+    EXPECT(r[0] == 0b00001 && r[1] == 0b00010 && r[2] == 0b00100 && r[3] == 0b01000 && r[4] == 0b10000);
+    EXPECT(r.all() == 0b11111);
+    EXPECT((*m)[0] == 0b00001 && (*m)[1] == 0b00100 && (*m)[2] == 0b01000 && m->all() == 0b01101);
+    EXPECT((*x)[0] == 0b00001 && (*x)[1] == 0b00010 && (*x)[2] == 0b00100 && x->all() == 0b00111);
+    EXPECT((*y)[0] == 0b01000 && (*y)[1] == 0b10000 && y->all() == 0b11000);
+}
+// quant.rs:603-623: a register of at most 8 amplitudes prints them all, without ".."
+static void debug_small() {
+    QReg q = QReg::with_state(1, 1);
+    EXPECT(q.debug() == "QReg { 0: Complex { re: 0.0, im: 0.0 }, 1: Complex { re: 1.0, im: 0.0 } }");
+    QReg z = QReg::new_(0);
+    EXPECT(z.debug() == "QReg { 0: Complex { re: 1.0, im: 0.0 } }");
 }
 // benches/performance.rs:4-11 (perf_test_single) at 18..20 qubits, plain and fused
 static void bench_workload() {
@@ -117,7 +193,12 @@ int main(int argc, char** argv) {
     multi_ops();
     readme_overlap();
     rotation_mask_panics();
+    creg_doc();
+    creg_bits();
+    vreg_index();
+    vreg_doc();
     if (mode == "device") {
+        debug_small();
         quantum_reg();
         bell();
         tensor_reg3();
