@@ -4,19 +4,19 @@
 // A request is one circuit on a fresh register followed by a batch of
 // non-destructive shots of measure_mask.  The server owns
 //   - one HBM arena (the admission budget), sub-allocated first-fit;
-//   - `slots` HIP streams, each with a completion event and pinned host
-//     staging for draws, fused programs and outcomes;
-//   - a scheduler thread.
+//   - `slots` HIP streams, each with a completion event, pinned host
+//     staging for draws, fused programs and outcomes, and a worker thread.
 // Requests queue at submit (validated, fused programs compiled on the
-// caller's thread).  The scheduler admits a queued request as soon as a slot
-// and arena space are free -- smaller requests may pass a large one at the
-// head of the queue, up to kMaxBypass times -- and enqueues its whole life
-// on the slot's stream: init, apply (fused or plain), measurement tree and
-// draws, outcome copy, event.  Requests on different slots overlap on the
-// device.  Completion is seen by polling the events; the arena block and
-// the slot are then reused at once.  Kernels are deterministic (fixed
-// reduction orders, no atomics), so a request's outcomes do not depend on
-// what runs beside it.
+// caller's thread).  A slot's worker admits the first queued request that
+// fits the arena -- smaller requests may pass a large one at the head of
+// the queue, up to kMaxBypass times -- and enqueues its whole life on the
+// slot's stream: init, apply (fused or plain), measurement tree and draws,
+// outcome copy, event.  It then waits for the event, returns the arena
+// block and posts the result.  Workers enqueue in parallel, so a long
+// request's thousands of launches do not hold up admission into the other
+// slots, and requests on different slots overlap on the device.  Kernels
+// are deterministic (fixed reduction orders, no atomics), so a request's
+// outcomes do not depend on what runs beside it.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -133,7 +133,7 @@ struct qvnt_server {
     std::deque<Record*> pending;
     std::deque<Record*> done;
     std::unordered_set<uint64_t> ids;  // ids submitted and not yet returned by poll
-    std::thread worker;
+    std::vector<std::thread> workers;  // one per slot
     bool stop = false;
     int device = 0;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -142,7 +142,8 @@ struct qvnt_server {
 
     int start(Record* rec, Slot& sl);
     void finish(Slot& sl, int status);
-    void loop();
+    Record* admit();
+    void work(Slot& sl);
 };
 
 // Runs without the server lock: touches only the slot, the record and the
@@ -244,66 +245,52 @@ void qvnt_server::finish(Slot& sl, int status) {
     done.push_back(rec);
 }
 
-void qvnt_server::loop() {
+// The first queued request that fits the arena (its block allocated), or
+// null; called with the lock held.  The head of the queue may be passed
+// over at most kMaxBypass times, then admission waits for space for it.
+Record* qvnt_server::admit() {
+    for (auto it = pending.begin(); it != pending.end(); ++it) {
+        Record* rec = *it;
+        size_t off = 0;
+        if (!arena.alloc(rec->bytes, &off)) {
+            if (it == pending.begin() && rec->bypassed >= kMaxBypass) return nullptr;
+            continue;
+        }
+        if (it != pending.begin()) pending.front()->bypassed++;
+        rec->arena_off = off;
+        rec->res.t_admit = now();
+        pending.erase(it);
+        return rec;
+    }
+    return nullptr;
+}
+
+// One slot's worker: admit, enqueue the request's whole life on the slot's
+// stream (without the lock), wait for its completion event, post the result.
+void qvnt_server::work(Slot& sl) {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        bool progressed = false;
-        bool busy = false;
-        for (Slot& sl : slots) {
-            if (!sl.rec) continue;
-            const hipError_t e = hipEventQuery(sl.done);
-            if (e == hipErrorNotReady) {
-                busy = true;
-                continue;
-            }
-            finish(sl, e == hipSuccess ? QVNT_OK : QVNT_E_HIP);
-            progressed = true;
+        Record* rec = nullptr;
+        cv_work.wait(lk, [&] { return stop || (!pending.empty() && (rec = admit()) != nullptr); });
+        if (!rec) {
+            if (stop) break;
+            continue;
         }
-        if (progressed) cv_done.notify_all();
-        // admission: the first queued request that fits, unless the head of
-        // the queue has been passed over too often
-        for (Slot& sl : slots) {
-            if (sl.rec || pending.empty()) continue;
-            bool admitted = false;
-            for (auto it = pending.begin(); it != pending.end(); ++it) {
-                Record* rec = *it;
-                size_t off = 0;
-                if (!arena.alloc(rec->bytes, &off)) {
-                    if (it == pending.begin() && rec->bypassed >= kMaxBypass) break;
-                    continue;
-                }
-                if (it != pending.begin()) pending.front()->bypassed++;
-                rec->arena_off = off;
-                rec->res.t_admit = now();
-                pending.erase(it);
-                sl.rec = rec;
-                lk.unlock();
-                const int rc = start(rec, sl);
-                if (rc != QVNT_OK) (void)hipStreamSynchronize(sl.stream);
-                lk.lock();
-                if (rc != QVNT_OK) {
-                    finish(sl, rc);
-                    cv_done.notify_all();
-                }
-                admitted = true;
-                busy = true;
-                break;
-            }
-            if (!admitted) break;
-        }
-        if (stop && pending.empty()) {
-            bool any = false;
-            for (Slot& sl : slots) any = any || sl.rec;
-            if (!any) break;
-        }
-        if (busy || !pending.empty()) {
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
-            lk.lock();
+        sl.rec = rec;
+        lk.unlock();
+        int rc = start(rec, sl);
+        if (rc == QVNT_OK) {
+            hipError_t e;
+            while ((e = hipEventQuery(sl.done)) == hipErrorNotReady) std::this_thread::sleep_for(std::chrono::microseconds(20));
+            if (e != hipSuccess) rc = QVNT_E_HIP;
         } else {
-            cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            (void)hipStreamSynchronize(sl.stream);
         }
+        lk.lock();
+        finish(sl, rc);
+        cv_done.notify_all();
+        cv_work.notify_all();   // arena space is free again
     }
 }
 
@@ -333,7 +320,7 @@ int qvnt_server_new(uint64_t hbm_bytes, uint32_t slots, qvnt_server** out) {
             return fail(QVNT_E_HIP, "stream or event creation failed");
         }
     }
-    s->worker = std::thread([s] { s->loop(); });
+    for (Slot& sl : s->slots) s->workers.emplace_back([s, &sl] { s->work(sl); });
     *out = s;
     return QVNT_OK;
 }
@@ -398,7 +385,7 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
         return QVNT_OK;
     }
     s->pending.push_back(rec);
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return QVNT_OK;
 }
 
@@ -440,7 +427,8 @@ int qvnt_server_free(qvnt_server* s) {
         s->pending.clear();
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (std::thread& w : s->workers)
+        if (w.joinable()) w.join();
     for (Record* rec : s->done) delete rec;
     for (Slot& sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
