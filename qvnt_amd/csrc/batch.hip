@@ -417,6 +417,15 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         //  1. one tile cap and one dense threshold for every size, over the grid;
         //  2. for each size that fits one tile, its own cap = n (one pass), kept
         //     if the whole batch gets faster.
+        // The timed runs evolve the registers: every way out of this branch,
+        // an error included, sets them back to their initial basis states.
+        struct ResetOnExit {
+            qvnt_batch* b;
+            bool armed = true;
+            ~ResetOnExit() {
+                if (armed) (void)reset(b);
+            }
+        } reset_guard{b};
         auto time_all = [&](const uint32_t* tc, const uint32_t* td, const std::vector<uint32_t>& recompile,
                             double* ms) -> int {
             compile_plans(gates, b->q, recompile, tc, td, (unsigned)b->amp(), plans, &bad);
@@ -467,6 +476,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
                 compile_plans(gates, b->q, kv.second, caps, dense, (unsigned)b->amp(), plans, &bad);  // back to the kept plans
             }
         }
+        reset_guard.armed = false;
         if (int rc = reset(b)) return rc;
     }
     compile_plans(gates, b->q, all, caps, dense, (unsigned)b->amp(), plans, &bad);

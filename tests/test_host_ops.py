@@ -142,3 +142,22 @@ def test_workload_op_counts():
     assert circuits.single_ops(layer) == 106 + 105
     assert len(circuits.build(layer, op)) == 211
     assert circuits.single_ops(circuits.tfim_step(34)) == 68
+
+
+def test_u2_controlled_on_its_b_qubit_is_accepted_by_c_and_refused_by_apply():
+    """The reference's U2 reports only a_mask in acts_on (atomic/u2.rs:70-72),
+    so .c(b_mask) succeeds on the operator (single/mod.rs:109-118), as here.
+    Applying it would make the reference's gather read a control it also
+    targets; this engine refuses it at apply time (QVNT_E_CTRL_OVERLAP,
+    before any device work), a deliberate deviation."""
+    import numpy as np
+    from qvnt_amd._lib import E_CTRL_OVERLAP
+    m = np.eye(4, dtype=np.complex128)[[1, 0, 3, 2]]      # a unitary 4x4 (swaps within pairs)
+    u = op.matrix2(0b001, 0b010, m)
+    assert u.act_on() == 0b001
+    cu = u.c(0b010)
+    assert cu is not None and cu.act_on() == 0b011
+    # the compiler of the fused form refuses it too (no device needed)
+    with pytest.raises(qvnt_amd.QvntError) as e:
+        qvnt_amd.fused_plan_stats(cu, 4)
+    assert e.value.code == E_CTRL_OVERLAP
