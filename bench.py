@@ -1,26 +1,48 @@
 #!/usr/bin/env python3
 """Benchmark: gates/s and % of HBM roofline of the plain FP64 apply on the
 30-qubit brickwork random circuit (16 GiB state), one MI355X, with the
-fused form on the same layers beside it (key "fused").
+fused form on the same layers beside it (key "fused"), and the rest of the
+metric in the same JSON line.
 
 A step is one brickwork layer: u3 on every qubit (3 SingleOps each), rzz on
 alternating nearest-neighbour pairs, one controlled x -- 105 or 106
-reference SingleOps.  --steps K layers are timed after --warmup W layers
-(default 40: the depth-40 circuit).  `value` counts reference SingleOps
-(u3 = 3), the unit the plain form launches one kernel for.
+reference SingleOps.  --steps K layers of the depth-40 circuit are timed
+after --warmup W layers.  `value` counts reference SingleOps (u3 = 3), the
+unit the plain form launches one kernel for.
 
 The roofline object reports the kernel class that dominates device time,
 from HIP events recorded on the register's stream around every launch in
 the timed region: achieved = algorithmic bytes 2 * 16 B * 2^(n - #controls)
 per launch / average launch time.  The CPU baseline times the oracle (the C
 restatement of qvnt's out-of-place gather with its OpenMP twin of the Rayon
-loop) on the host on a bounded sample of the same workload.
+loop) on the host on a bounded sample of the same workload, plus the
+20-qubit case and the reference's own benchmark workload.
+
+Further keys of the default run (skipped with --no-extra), each leg freeing
+its device memory before the next:
+  tfim       TFIM ring, 34 qubits FP64 and 35 qubits FP32 (256 GiB each):
+             one plain Trotter step with its dominant kernel's roofline, and
+             50 steps through the fused form as one call;
+  tfim_pair  28 qubits, FP32 against FP64 after every one of 50 steps
+             (fused and plain): max |dpsi| and infidelity;
+  batch      4096 registers of 10-20 qubits, FP64, autotuned and run as one
+             hipGraph: gates/s, ms per run, HBM and matrix-core rates;
+  served     128 serving requests (12-30 qubits, 100-2000 gates, 1024
+             shots): one-at-a-time plain and fused loops, the
+             continuous-batching server at saturation, and open-loop
+             Poisson arrivals at 50/80/100 % of the serial fused capacity
+             into a one-slot and a four-slot server (p50/p99 latency).
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+# The CPU baseline's OpenMP threads, one per core; set before the oracle's
+# OpenMP runtime starts.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -34,33 +56,117 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, budget_s):
-    """Oracle (qvnt's algorithm in C, OpenMP) on the first SingleOps of
-    brickwork layer 0 at full size, grown until ~budget_s of work."""
+def host_info():
+    """The host the CPU baseline runs on: lscpu model, sockets, physical
+    cores, NUMA nodes; the CPUs this process may use; OpenMP pinning."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        sockets = int(kv.get("Socket(s)", "1") or 1)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info = {"model": kv.get("Model name"), "sockets": sockets, "cores_per_socket": cores,
+                "physical_cores": sockets * cores, "threads_per_core": int(kv.get("Thread(s) per core", "1") or 1),
+                "numa_nodes": int(kv.get("NUMA node(s)", "1") or 1)}
+    except (OSError, ValueError, subprocess.SubprocessError) as e:
+        info = {"lscpu_error": str(e)}
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    info["omp_num_threads_env"] = os.environ.get("OMP_NUM_THREADS")
+    info["pinning"] = "OMP_PROC_BIND=%s OMP_PLACES=%s" % (os.environ.get("OMP_PROC_BIND"), os.environ.get("OMP_PLACES"))
+    return info
+
+
+def cpu_threads(host):
+    """Threads for the CPU baseline: one per physical core this process may
+    use, capped by OMP_NUM_THREADS when the environment sets it (a GPU lease
+    grants a share of the host's cores: 16 per GPU on the MI355X boxes)."""
+    n = min(host.get("affinity_cpus") or 1, host.get("physical_cores") or host.get("affinity_cpus") or 1)
+    env = host.get("omp_num_threads_env")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(n, threads, ks=(1, 4)):
+    """The oracle (qvnt's algorithm in C, OpenMP twin of the Rayon loop) on
+    the brickwork at full size: the first k SingleOps of layer 0 as one
+    MultiOp::apply, for each k in ks.  Each apply pays the reference's
+    per-apply costs (a fresh output buffer and a copy of the input,
+    multi/mod.rs:98, quant.rs:382-383) plus one out-of-place gather per
+    SingleOp, so time = a + b k is fitted to separate the two; the layer rate
+    is 106 / (a + 106 b), the reference applying a whole layer as one
+    MultiOp."""
     import oracle
     from qvnt_amd import circuits
-    threads = min(os.cpu_count() or 1, 16)
     specs = circuits.brickwork_layer(n, 0)
     full = circuits.build(specs, oracle.op)
-    reg = oracle.Reg(n, 0, threads=threads if threads > 1 else 1)
-    done, elapsed, count = 0, 0.0, 1
-    while elapsed < budget_s and done < len(full.gates):
-        count = min(count, len(full.gates) - done)
-        part = oracle.Op(full.gates[done:done + count])
+    reg = oracle.Reg(n, 0, threads=threads)
+    runs = []
+    for k in ks:
+        part = oracle.Op(full.gates[:k])
         t0 = time.perf_counter()
         reg.apply(part)
-        dt = time.perf_counter() - t0
-        elapsed += dt
-        done += count
-        per = dt / count
-        count = max(1, int((budget_s - elapsed) / per / 2)) if per > 0 else 1
+        runs.append((k, time.perf_counter() - t0))
     del reg
+    done = sum(k for k, _ in runs)
+    elapsed = sum(t for _, t in runs)
+    (k0, t0_), (k1, t1_) = runs[0], runs[-1]
+    b = (t1_ - t0_) / (k1 - k0) if k1 != k0 else t1_ / k1
+    a = max(0.0, t0_ - b * k0)
+    layer = len(full.gates)
     return {"value": done / elapsed, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": "first %d SingleOps of brickwork layer 0 at %d qubits (FP64, one MultiOp::apply per batch, "
-                      "out-of-place gather + per-apply copy), %.1f s" % (done, n, elapsed)}
+            "sample": "first %s SingleOps of brickwork layer 0 at %d qubits, one MultiOp::apply each (FP64, "
+                      "out-of-place gather + per-apply copy), %.1f s" % ("/".join(str(k) for k, _ in runs), n, elapsed),
+            "apply_seconds": {str(k): round(t, 3) for k, t in runs},
+            "per_singleop_s": round(b, 4), "per_apply_overhead_s": round(a, 3),
+            "layer_as_one_apply_gates_per_s": round(layer / (a + layer * b), 4) if a + layer * b > 0 else None}
 
 
-def config1_leg():
+def reference_bench_leg(threads):
+    """The reference's own benchmark (benches/performance.rs:4-20): per
+    iteration QReg::with_state(q, 0), apply(qft(0b0111) * qft(0b1110)),
+    measure_mask(0b100), for q = 18, 19, 20 -- on the GPU (plain form,
+    register created and freed every iteration, as the reference allocates
+    its Vec) and on the oracle on 1 and `threads` host threads.  Iterations
+    per second; each side runs for about half a second."""
+    import oracle
+    from qvnt_amd import QReg, op, uniform
+    out = {}
+    mask = 0b100
+    for q in (18, 19, 20):
+        gops = op.qft(0b0111) * op.qft(0b1110)
+        cops = oracle.op.qft(0b0111) * oracle.op.qft(0b1110)
+        row = {}
+        it, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 0.5 or it < 3:
+            with QReg.with_state(q, 0) as r:
+                r.apply(gops)
+                got = r.measure_mask(mask, u=uniform(0x51564E54, q, it)).get()
+            assert got & ~mask == 0
+            it += 1
+        row["gpu_iter_per_s"] = round(it / (time.perf_counter() - t0), 1)
+        for th in sorted({threads, 1}):
+            it, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 0.5 or it < 3:
+                ref = oracle.Reg(q, 0, threads=th)
+                ref.apply(cops)
+                got = ref.measure_mask(mask, uniform(0x51564E54, q, it))
+                assert got & ~mask == 0
+                del ref
+                it += 1
+            row["cpu_port_iter_per_s_%dthreads" % th] = round(it / (time.perf_counter() - t0), 1)
+        out["q%d" % q] = row
+        log("reference bench q=%d: %s" % (q, row))
+    return {"workload": "benches/performance.rs: with_state(q,0); apply(qft(0b0111)*qft(0b1110)); "
+                        "measure_mask(0b100); iterations/s", "single_ops": 12, "by_qubits": out}
+
+
+def config1_leg(threads):
     """The CPU-runnable case (20 qubits: qft on all qubits, h on all, 200
     random gates; the state fits the CPU caches, so the oracle is not
     page-fault bound here): the plain GPU form and the fused form (compiled
@@ -102,11 +208,11 @@ def config1_leg():
         fused = reps * count / (time.perf_counter() - t0)
     cpu = {}
     cops = circuits.build(specs, oracle.op)
-    for threads in sorted({min(os.cpu_count() or 1, 16), 1}, reverse=True):
-        ref = oracle.Reg(n, state, threads=threads)
+    for th in sorted({threads, 1}, reverse=True):
+        ref = oracle.Reg(n, state, threads=th)
         t0 = time.perf_counter()
         ref.apply(cops)
-        cpu[str(threads)] = round(count / (time.perf_counter() - t0), 1)
+        cpu[str(th)] = round(count / (time.perf_counter() - t0), 1)
         del ref
     return {"workload": "plain-20q-fp64: qft(2^20-1), h on all qubits, 200 random gates (%d SingleOps)" % count,
             "gpu_plain_gates_per_s": round(gpu, 1), "gpu_fused_gates_per_s": round(fused, 1),
@@ -156,206 +262,241 @@ def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value, dense=0):
     return out
 
 
-def tfim(args):
+def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0):
     """Transverse-field Ising Trotter steps on a ring (rzz on every edge, then
-    rx on every qubit; J = 1, dt = 0.05, h_i ~ U[0.5, 1.5]); a step is one
-    Trotter step of 2n SingleOps.  The plain form runs --steps steps; the
-    fused form runs all --fused-steps steps, one call per step and then the
-    whole evolution as one call."""
+    rx on every qubit; J = 1, dt = 0.05, h_i ~ U[0.5, 1.5]); a step is 2n
+    SingleOps.  The plain form runs plain_steps steps; the fused form runs
+    fused_steps steps as one call (and, with per_step, one call per step).
+    One register serves every form (reset to |0> in between)."""
     from qvnt_amd import QReg, circuits, op
-    n, prec = args.qubits, args.prec
     step = circuits.build(circuits.tfim_step(n), op)
     amp = 16 if prec == "f64" else 8
-    out = {"plain": None, "fused": {}}
+    out = {"workload": "tfim-ring-%dq-%s" % (n, prec), "qubits": n, "state_gib": (amp << n) / 2**30,
+           "single_ops_per_step": len(step)}
     log("tfim: %d qubits %s (%.1f GiB)" % (n, prec, (amp << n) / 2**30))
-    if args.steps > 0 and not args.no_plain:
-        reg = QReg(n, prec=prec)
-        reg.set_timing(True)
-        reg.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            reg.apply(step)
-        reg.sync()
-        wall = time.perf_counter() - t0
-        st = reg.kernel_stats(reset=True)
-        norm = reg.get_absolute()
+    reg = QReg(n, prec=prec)
+    try:
+        if plain_steps > 0:
+            reg.set_timing(True)
+            reg.sync()
+            t0 = time.perf_counter()
+            for _ in range(plain_steps):
+                reg.apply(step)
+            reg.sync()
+            wall = time.perf_counter() - t0
+            st = reg.kernel_stats(reset=True)
+            reg.set_timing(False)
+            norm = reg.get_absolute()
+            gates = plain_steps * len(step)
+            dom = max((k for k in st if st[k]["launches"]), key=lambda k: st[k]["ms"])
+            d = st[dom]
+            ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+            out["plain"] = {"value": round(gates / wall, 3), "unit": "gates/s", "steps": plain_steps,
+                            "seconds": round(wall, 3), "norm_minus_1": norm - 1.0,
+                            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                         "kernel": "k_" + dom, "launches": d["launches"],
+                                         "avg_ms": round(d["ms"] / d["launches"], 3)}}
+            log("tfim plain: %d SingleOps in %.2f s -> %.2f gates/s, k_%s %.0f GB/s; |psi|^2-1 = %.1e"
+                % (gates, wall, gates / wall, dom, ach, norm - 1.0))
+        out["fused"] = {}
+        forms = ([("per_step", 1)] if per_step else []) + [("whole", fused_steps)]
+        for label, per_call in forms if fused_steps > 0 else ():
+            full = step.clone()
+            for _ in range(per_call - 1):
+                full *= step
+            reg.reset(0)
+            reg.set_dense(dense)
+            reg.set_timing(True)
+            reg.sync()
+            reg.kernel_stats(reset=True)
+            passes = 0
+            t0 = time.perf_counter()
+            for _ in range(fused_steps // per_call):
+                passes += reg.apply_fused(full)
+            reg.sync()
+            wall = time.perf_counter() - t0
+            st = reg.kernel_stats(reset=True)
+            reg.set_timing(False)
+            norm = reg.get_absolute()
+            gates = fused_steps * len(step)
+            f = st["fused"]
+            ach = f["bytes"] / (f["ms"] * 1e-3) / 1e9 if f["ms"] else 0.0
+            plain_v = out.get("plain", {}).get("value")
+            out["fused"][label] = {"value": round(gates / wall, 3), "unit": "gates/s", "steps": fused_steps,
+                                   "passes": passes, "seconds": round(wall, 3),
+                                   "speedup_vs_plain": round(gates / wall / plain_v, 2) if plain_v else None,
+                                   "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                                "kernel": "k_fused", "launches": f["launches"],
+                                                "avg_ms": round(f["ms"] / f["launches"], 3) if f["launches"] else None},
+                                   "norm_minus_1": norm - 1.0}
+            log("tfim fused (%s): %d steps, %d passes, %.2f s -> %.1f gates/s, k_fused %.0f GB/s; |psi|^2-1 = %.1e"
+                % (label, fused_steps, passes, wall, gates / wall, ach, norm - 1.0))
+    finally:
         reg.close()
-        gates = args.steps * len(step)
-        dom = max((k for k in st if st[k]["launches"]), key=lambda k: st[k]["ms"])
-        d = st[dom]
-        ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-        out["plain"] = {"value": round(gates / wall, 3), "steps": args.steps, "seconds": round(wall, 3),
-                        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                     "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": dom,
-                                     "avg_ms": round(d["ms"] / d["launches"], 3)}}
-        log("tfim plain: %d SingleOps in %.2f s -> %.2f gates/s, %s %.0f GB/s; |psi|^2-1 = %.1e"
-            % (gates, wall, gates / wall, dom, ach, norm - 1.0))
-    for label, per_call in (("per_step", 1), ("whole", args.fused_steps)) if args.fused_steps > 0 else ():
-        full = step.clone()
-        for _ in range(per_call - 1):
-            full *= step
-        reg = QReg(n, prec=prec)
-        reg.set_dense(args.dense)
-        reg.set_timing(True)
-        reg.sync()
-        passes = 0
-        t0 = time.perf_counter()
-        for _ in range(args.fused_steps // per_call):
-            passes += reg.apply_fused(full)
-        reg.sync()
-        wall = time.perf_counter() - t0
-        st = reg.kernel_stats(reset=True)["fused"]
-        norm = reg.get_absolute()
-        reg.close()
-        gates = args.fused_steps * len(step)
-        ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9
-        out["fused"][label] = {"value": round(gates / wall, 3), "passes": passes, "seconds": round(wall, 3),
-                               "k_fused_gbs": round(ach, 1), "norm_minus_1": norm - 1.0}
-        log("tfim fused (%s): %d steps, %d passes, %.2f s -> %.1f gates/s, k_fused %.0f GB/s; |psi|^2-1 = %.1e"
-            % (label, args.fused_steps, passes, wall, gates / wall, ach, norm - 1.0))
-    line = {"metric": METRIC, "value": out["plain"]["value"] if out["plain"] else None, "unit": "gates/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": prec, "data": "synthetic (seeded TFIM ring)",
-            "config": {"workload": "tfim-ring-%dq-%s" % (n, prec), "qubits": n, "state_gib": (amp << n) / 2**30,
-                       "fused_steps": args.fused_steps, "dense_min": args.dense},
-            "plain": out["plain"], "fused": out["fused"]}
-    print(json.dumps(line), flush=True)
+    return out
 
 
-def tfim_pair(args):
+def tfim_pair_leg(n, steps, forms=("fused", "plain")):
     """The same TFIM evolution in an FP64 and an FP32 register side by side;
-    after every step the device compares them (max |dpsi|, infidelity)."""
+    after every step the device compares them in FP64 (max |dpsi|,
+    infidelity 1 - |<a|b>|^2 / (|a|^2 |b|^2))."""
     from qvnt_amd import QReg, circuits, op
-    n = args.qubits
     step = circuits.build(circuits.tfim_step(n), op)
-    r64, r32 = QReg(n, prec="f64"), QReg(n, prec="f32")
-    rows = []
-    for k in range(args.fused_steps):
-        if args.no_fused:
-            r64.apply(step)
-            r32.apply(step)
-        else:
-            r64.apply_fused(step)
-            r32.apply_fused(step)
-        err, infid, _, n64, n32 = r64.compare(r32)
-        rows.append({"step": k + 1, "max_abs_err": err, "infidelity": infid, "norm64_minus_1": n64 - 1.0,
-                     "norm32_minus_1": n32 - 1.0})
-        log("step %2d: max|dpsi| %.3e  infidelity %.3e" % (k + 1, err, infid))
-    print(json.dumps({"config": {"workload": "tfim-ring-%dq fp32 vs fp64" % n, "form": "plain" if args.no_fused else "fused"},
-                      "steps": rows}), flush=True)
+    out = {"workload": "tfim-ring-%dq fp32 vs fp64, %d steps" % (n, steps)}
+    for form in forms:
+        r64, r32 = QReg(n, prec="f64"), QReg(n, prec="f32")
+        rows = []
+        try:
+            for k in range(steps):
+                if form == "plain":
+                    r64.apply(step)
+                    r32.apply(step)
+                else:
+                    r64.apply_fused(step)
+                    r32.apply_fused(step)
+                err, infid, _, n64, n32 = r64.compare(r32)
+                rows.append([k + 1, err, infid])
+        finally:
+            r64.close()
+            r32.close()
+        out[form] = {"max_abs_err_per_step": [r[1] for r in rows], "infidelity_per_step": [r[2] for r in rows],
+                     "max_abs_err": max(r[1] for r in rows), "max_infidelity": max(r[2] for r in rows),
+                     "final_max_abs_err": rows[-1][1], "final_infidelity": rows[-1][2]}
+        log("tfim pair %d q (%s): max |dpsi| %.3e, max infidelity %.3e over %d steps"
+            % (n, form, out[form]["max_abs_err"], out[form]["max_infidelity"], steps))
+    return out
 
 
-def serve(args):
-    """Serving workload: --requests circuits (n log-uniform in [12, 30],
-    100-2000 mixed gates, 1024 seeded shots of measure_mask on 8 random
-    qubits), three ways:
-      serial_plain  one request at a time, plain form (a closed loop: the next
-                    request is submitted when the last has finished);
-      serial_fused  one request at a time, fused form;
-      served        Poisson arrivals at --rate requests/s into the
-                    continuous-batching server (--slots streams, an HBM arena
-                    of --arena-gib), fused form.
+def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0.8, 1.0)):
+    """Serving workload (n log-uniform in [12, n_hi], 100-2000 mixed gates,
+    1024 seeded shots of measure_mask on 8 random qubits per request):
+      serial_plain   closed loop, one request at a time, plain form;
+      serial_fused   closed loop, one request at a time, fused form (its
+                     circuits/s is the capacity c of serial service);
+      saturation     every request submitted at once into the
+                     continuous-batching server (`slots` streams);
+      open_loop      the same Poisson arrival times at load x c for each
+                     load, into a one-slot server (serial fused service,
+                     first come first served) and into the `slots` server.
     Latency = completion - submission (submission includes validation and
     the fused-program compile); circuits/s = requests / (last completion -
-    first submission)."""
+    first submission).  Every fused run must give the same outcomes."""
     import numpy as np
     from qvnt_amd import circuits, op
     from qvnt_amd.serve import Server
-    reqs = circuits.serving_requests(args.requests, n_lo=12, n_hi=args.qubits)
+    reqs = circuits.serving_requests(requests, n_lo=12, n_hi=n_hi)
     t0 = time.perf_counter()
     ops = [circuits.build(specs, op) for _, specs, _, _ in reqs]
-    log("serve: %d requests built in %.1f s; qubits %s; gates %d..%d" % (
-        len(reqs), time.perf_counter() - t0, np.bincount([r[0] for r in reqs]).nonzero()[0].tolist(),
+    qs = [r[0] for r in reqs]
+    log("serve: %d requests built in %.1f s; qubits %d..%d (%d of >= 28); gates %d..%d" % (
+        len(reqs), time.perf_counter() - t0, min(qs), max(qs), sum(q >= 28 for q in qs),
         min(len(o) for o in ops), max(len(o) for o in ops)))
-    arena = int(args.arena_gib * 2**30)
+    arena = int(arena_gib * 2**30)
 
     def stats(res, label):
+        bad = [r["id"] for r in res if r["status"] != 0]
+        if bad or len(res) != len(reqs):
+            raise RuntimeError("%s: %d results, failed requests %s" % (label, len(res), bad[:8]))
         lat = np.array([r["t_done"] - r["t_submit"] for r in res])
         span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
-        bad = [r["id"] for r in res if r["status"] != 0]
-        if bad:
-            raise RuntimeError("%s: requests %s failed" % (label, bad[:8]))
-        out = {"circuits_per_s": round(len(res) / span, 3), "seconds": round(span, 3),
-               "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
-               "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
-               "mean_ms": round(1e3 * float(lat.mean()), 2)}
-        log("%s: %.2f circuits/s, p50 %.1f ms, p99 %.1f ms (%.1f s)" % (label, out["circuits_per_s"], out["p50_ms"],
-                                                                        out["p99_ms"], span))
-        return out
+        o = {"circuits_per_s": round(len(res) / span, 3), "seconds": round(span, 3),
+             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+             "mean_ms": round(1e3 * float(lat.mean()), 2), "max_ms": round(1e3 * float(lat.max()), 2)}
+        log("%s: %.2f circuits/s, p50 %.1f ms, p99 %.1f ms (%.1f s)" % (label, o["circuits_per_s"], o["p50_ms"],
+                                                                        o["p99_ms"], span))
+        return o
 
-    def closed_loop(fused, slots=1):
+    def closed_loop(fused):
         res = []
-        with Server(arena, slots=slots) as srv:
+        with Server(arena, slots=1) as srv:
             for k, (n, _, mask, draws) in enumerate(reqs):
-                srv.submit(k, n, ops[k], mask, draws, fused=fused, dense_min=args.dense if fused else 0)
+                srv.submit(k, n, ops[k], mask, draws, fused=fused, dense_min=dense if fused else 0)
                 res += srv.drain(1)
         return res
 
-    out = {}
-    totals = {}
-    if not args.no_plain:
+    def open_loop(n_slots, gaps):
+        with Server(arena, slots=n_slots) as srv:
+            start = srv.now()
+            due = start + np.cumsum(gaps) - gaps[0]
+            for k, (n, _, mask, draws) in enumerate(reqs):
+                wait = due[k] - srv.now()
+                if wait > 0:
+                    time.sleep(wait)
+                srv.submit(k, n, ops[k], mask, draws, fused=True, dense_min=dense)
+            res = srv.drain(len(reqs))
+            peak = srv.pending()[2]
+        return res, peak
+
+    out = {"workload": "serving-%d-requests-12..%dq-100..2000-gates-1024-shots" % (len(reqs), n_hi),
+           "requests": len(reqs), "slots": slots, "arena_gib": arena_gib, "dense_min": dense,
+           "note": "p99 of %d latencies" % len(reqs)}
+    outcomes = {}
+
+    def same(res):
+        return all(np.array_equal(r["outcomes"], outcomes[r["id"]][0]) and r["total"] == outcomes[r["id"]][1]
+                   for r in res)
+
+    totals_plain = None
+    if plain:
         res = closed_loop(False)
         out["serial_plain"] = stats(res, "serial plain")
-        totals["plain"] = {r["id"]: r["total"] for r in res}
+        totals_plain = {r["id"]: r["total"] for r in res}
     res = closed_loop(True)
     out["serial_fused"] = stats(res, "serial fused")
-    totals["fused"] = {r["id"]: r["total"] for r in res}
-    serial_outcomes = {r["id"]: r["outcomes"] for r in res}
-    # open loop: Poisson arrivals at the offered rate
-    gaps = np.random.default_rng(circuits.SEED).exponential(1.0 / args.rate, len(reqs))
-    with Server(arena, slots=args.slots) as srv:
-        start = srv.now()
-        due = start + np.cumsum(gaps) - gaps[0]
-        for k, (n, _, mask, draws) in enumerate(reqs):
-            wait = due[k] - srv.now()
-            if wait > 0:
-                time.sleep(wait)
-            srv.submit(k, n, ops[k], mask, draws, fused=True, dense_min=args.dense)
-        res = srv.drain(len(reqs))
-        _, _, peak = srv.pending()
-    out["served"] = stats(res, "served (rate %.1f/s, %d slots)" % (args.rate, args.slots))
-    out["served"]["offered_rate"] = args.rate
-    out["served"]["slots"] = args.slots
-    out["served"]["arena_peak_gib"] = round(peak / 2**30, 2)
-    same = all(np.array_equal(r["outcomes"], serial_outcomes[r["id"]]) and r["total"] == totals["fused"][r["id"]]
-               for r in res)
-    out["served"]["outcomes_equal_serial_fused"] = bool(same)
-    if "plain" in totals:
-        out["max_total_diff_fused_vs_plain"] = max(abs(totals["fused"][k] - totals["plain"][k]) for k in totals["plain"])
-    base = out.get("serial_plain", out["serial_fused"])["circuits_per_s"]
-    line = {"metric": METRIC, "value": out["served"]["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
-            "steps": len(reqs), "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded serving requests)",
-            "config": {"workload": "serving-%d-requests-12..%dq-100..2000-gates-1024-shots" % (len(reqs), args.qubits),
-                       "arena_gib": args.arena_gib, "dense_min": args.dense},
-            "speedup_vs_serial_plain": round(out["served"]["circuits_per_s"] / base, 2) if base else None}
-    line.update(out)
-    print(json.dumps(line), flush=True)
+    outcomes = {r["id"]: (r["outcomes"], r["total"]) for r in res}
+    if totals_plain:
+        out["max_total_diff_fused_vs_plain"] = max(abs(outcomes[k][1] - totals_plain[k]) for k in totals_plain)
+    equal = True
+    res, peak = open_loop(slots, np.zeros(len(reqs)))
+    out["saturation"] = stats(res, "served, all at once (%d slots)" % slots)
+    out["saturation"]["arena_peak_gib"] = round(peak / 2**30, 2)
+    equal = equal and same(res)
+    cap = out["serial_fused"]["circuits_per_s"]
+    unit_gaps = np.random.default_rng(circuits.SEED).exponential(1.0, len(reqs))
+    out["open_loop"] = []
+    for load in loads:
+        rate = load * cap
+        row = {"load": load, "offered_rate": round(rate, 3)}
+        for label, n_slots in (("serial_fused", 1), ("served", slots)):
+            res, _ = open_loop(n_slots, unit_gaps / rate)
+            row[label] = stats(res, "%s at %.0f%% of serial capacity (%.1f/s)" % (label, 100 * load, rate))
+            equal = equal and same(res)
+        out["open_loop"].append(row)
+    out["outcomes_equal_across_runs"] = bool(equal)
+    if not equal:
+        raise RuntimeError("served outcomes differ between runs")
+    base = out.get("serial_plain", {}).get("circuits_per_s")
+    out["saturation_vs_serial_plain"] = round(out["saturation"]["circuits_per_s"] / base, 2) if base else None
+    out["saturation_vs_serial_fused"] = round(out["saturation"]["circuits_per_s"] / cap, 3)
+    return out
 
 
-def batch(args):
-    """Batched uneven registers: --registers registers (n uniform in [10, 20],
-    20-400 mixed gates, seeded), each from |0>.  The plain form applies every
-    register's circuit one kernel per SingleOp, register after register; the
-    batched form compiles all circuits (autotuned tile caps) and runs them
-    round by round, as one hipGraph, --steps times.  value = SingleOps/s of
-    the batched form over all registers."""
+def batch_leg(registers, n_hi, prec, tile_cap, dense, steps, warmup, check, plain=True):
+    """Batched uneven registers (n uniform in [10, n_hi], 20-400 mixed gates,
+    seeded), each from |0>.  The plain form applies every register's circuit
+    one kernel per SingleOp, register after register; the batched form
+    compiles all circuits (tile_cap 0: autotuned tile caps and dense-phase
+    thresholds) and runs them round by round as one hipGraph, `steps` runs
+    timed.  value = SingleOps/s of the batched form over all registers."""
     import numpy as np
     from qvnt_amd import QReg, circuits, op
     from qvnt_amd.batch import Batch
-    prec = args.prec
     t0 = time.perf_counter()
-    regs = circuits.batched_registers(args.registers, n_lo=10, n_hi=args.qubits)
+    regs = circuits.batched_registers(registers, n_lo=10, n_hi=n_hi)
     ops = [circuits.build(specs, op) for _, specs in regs]
     single = sum(len(o) for o in ops)
     amp = 16 if prec == "f64" else 8
     state_gib = sum(amp << n for n, _ in regs) / 2**30
     log("batch: %d registers, %d SingleOps, %.2f GiB of %s state (built in %.1f s)" % (
         len(regs), single, state_gib, prec, time.perf_counter() - t0))
-    out = {}
-    sample = min(args.check_registers, len(regs))
+    out = {"workload": "batched-%d-registers-10..%dq-20..400-gates-%s" % (len(regs), n_hi, prec),
+           "registers": len(regs), "single_ops": single, "state_gib": round(state_gib, 2)}
+    sample = min(check, len(regs))
     plain_states = []
-    if not args.no_plain:
+    if plain:
         # every register, in chunks of 512 live registers (allocation outside the timed spans)
         wall = 0.0
         for c0 in range(0, len(regs), 512):
@@ -371,14 +512,14 @@ def batch(args):
             plain_states += [live[k - c0].get_state()[: 1 << regs[k][0]] for k in range(c0, min(sample, c0 + 512))]
             for r in live:
                 r.close()
-        out["plain"] = {"value": round(single / wall, 1), "seconds": round(wall, 3)}
+        out["plain"] = {"value": round(single / wall, 1), "unit": "gates/s", "seconds": round(wall, 3)}
         log("plain: %.2f s -> %.0f SingleOps/s" % (wall, single / wall))
     with Batch([n for n, _ in regs], prec=prec) as b:
         t0 = time.perf_counter()
-        info = b.compile(ops, max_qubits=args.tile_cap, dense_min=args.dense)
+        info = b.compile(ops, max_qubits=tile_cap, dense_min=dense)
         comp = time.perf_counter() - t0
         log("compile%s: %.2f s, %d rounds, %d launches, %d passes, %d of %d phases dense; caps %s; dense %s" % (
-            " + autotune" if not args.tile_cap else "", comp, info["rounds"], info["launches"], info["passes"],
+            " + autotune" if not tile_cap else "", comp, info["rounds"], info["launches"], info["passes"],
             info["dense_phases"], info["phases"], info["tile_cap"], info["dense_min"]))
         b.run(graph=True)
         b.sync()
@@ -387,35 +528,72 @@ def batch(args):
             got = b.get_state(k)
             if plain_states:
                 errs.append(float(np.max(np.abs(got.astype(np.complex128) - plain_states[k]))))
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             b.run(graph=True)
-        ms_graph = b.time(args.steps, graph=True)
-        ms_eager = b.time(max(1, args.steps // 2), graph=False)
+        ms_graph = b.time(steps, graph=True)
+        ms_eager = b.time(max(1, steps // 2), graph=False)
     gbs = info["bytes"] / (ms_graph * 1e-3) / 1e9
     value = single / (ms_graph * 1e-3)
-    log("batched: %.2f ms/run (graph), %.2f ms/run (eager) -> %.0f SingleOps/s, %.0f GB/s algorithmic" % (
-        ms_graph, ms_eager, value, gbs))
-    out["batched"] = {"value": round(value, 1), "ms_per_run_graph": round(ms_graph, 3),
-                      "ms_per_run_eager": round(ms_eager, 3), "compile_s": round(comp, 2),
+    tflops = info["dense_flops"] / (ms_graph * 1e-3) / 1e12
+    log("batched: %.2f ms/run (graph), %.2f ms/run (eager) -> %.0f SingleOps/s, %.0f GB/s algorithmic, "
+        "%.1f TFLOP/s on the matrix cores" % (ms_graph, ms_eager, value, gbs, tflops))
+    out["batched"] = {"value": round(value, 1), "unit": "gates/s", "ms_per_run_graph": round(ms_graph, 3),
+                      "ms_per_run_eager": round(ms_eager, 3), "runs_timed": steps, "compile_s": round(comp, 2),
                       "rounds": info["rounds"], "launches": info["launches"], "passes": info["passes"],
                       "tiles": info["tiles"], "phases": info["phases"], "dense_phases": info["dense_phases"],
                       "tile_cap": info["tile_cap"], "dense_min": info["dense_min"], "tune_ms": info["tune_ms"],
                       "single_ms": info["single_ms"],
-                      "max_abs_err_vs_plain": max(errs) if errs else None, "checked_registers": len(errs)}
-    line = {"metric": METRIC, "value": round(value, 1), "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": prec, "data": "synthetic (seeded batched registers)",
-            "config": {"workload": "batched-%d-registers-10..%dq-20..400-gates-%s" % (len(regs), args.qubits, prec),
-                       "registers": len(regs), "single_ops": single, "state_gib": round(state_gib, 2)},
-            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch"},
-            "mfma": {"achieved": round(info["dense_flops"] / (ms_graph * 1e-3) / 1e12, 3),
-                     "peak": MFMA_PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                     "frac": round(info["dense_flops"] / (ms_graph * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS[prec], 4),
-                     "flops_per_run": info["dense_flops"], "note": "dense 16x16 phases (real 32x32 GEMMs) only"},
-            "speedup_vs_plain": round(value / out["plain"]["value"], 2) if "plain" in out else None}
-    line.update(out)
-    print(json.dumps(line), flush=True)
+                      "max_abs_err_vs_plain": max(errs) if errs else None, "checked_registers": len(errs),
+                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": "k_batch",
+                                   "bytes_per_run": info["bytes"]},
+                      "mfma": {"achieved": round(tflops, 3), "peak": MFMA_PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                               "frac": round(tflops / MFMA_PEAK_TFLOPS[prec], 4), "flops_per_run": info["dense_flops"],
+                               "note": "dense 16x16 phases (real 32x32 GEMMs) only"}}
+    if "plain" in out:
+        out["batched"]["speedup_vs_plain"] = round(value / out["plain"]["value"], 2)
+    return out
+
+
+def _line(workload_line, **extra):
+    line = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None}
+    line.update(workload_line)
+    line.update(extra)
+    return line
+
+
+def tfim(args):
+    res = tfim_leg(args.qubits, args.prec, args.steps, args.fused_steps, per_step=True, dense=args.dense)
+    v = res.get("plain", {}).get("value")
+    print(json.dumps(_line({"value": v, "unit": "gates/s", "steps": args.steps, "warmup": 0, "dtype": args.prec,
+                            "data": "synthetic (seeded TFIM ring)",
+                            "config": {"workload": res["workload"], "qubits": args.qubits,
+                                       "fused_steps": args.fused_steps, "dense_min": args.dense}}, tfim=res)),
+          flush=True)
+
+
+def tfim_pair(args):
+    forms = ("plain",) if args.no_fused else ("fused", "plain")
+    res = tfim_pair_leg(args.qubits, args.fused_steps, forms)
+    print(json.dumps({"config": {"workload": res["workload"]}, "tfim_pair": res}), flush=True)
+
+
+def serve(args):
+    res = serve_leg(args.requests, args.qubits, args.slots, args.arena_gib, args.dense, plain=not args.no_plain)
+    print(json.dumps(_line({"value": res["saturation"]["circuits_per_s"], "unit": "circuits/s",
+                            "steps": res["requests"], "warmup": 0, "dtype": "f64",
+                            "data": "synthetic (seeded serving requests)",
+                            "config": {"workload": res["workload"]}}, served=res)), flush=True)
+
+
+def batch(args):
+    res = batch_leg(args.registers, args.qubits, args.prec, args.tile_cap, args.dense, args.steps, args.warmup,
+                    args.check_registers, plain=not args.no_plain)
+    b = res["batched"]
+    print(json.dumps(_line({"value": b["value"], "unit": "gates/s", "steps": args.steps, "warmup": args.warmup,
+                            "ms_per_step": b["ms_per_run_graph"], "dtype": args.prec,
+                            "data": "synthetic (seeded batched registers)", "config": {"workload": res["workload"]},
+                            "roofline": b["roofline"]}, batch=res)), flush=True)
 
 
 def load_traffic(kernel):
@@ -487,14 +665,15 @@ def tiles(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--qubits", type=int, default=None,
                     help="register size (largest size for serve/batch); default per workload")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="brickwork: skip the served / TFIM / FP32-vs-FP64 / batched legs")
     ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
@@ -502,8 +681,7 @@ def main():
                     help="fused forms: fold phases of >= this many ops into dense 16x16 unitaries (0: none; "
                          "default 0, serve 3; batch: used with --tile-cap, else autotuned)")
     ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
-    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=20.0, help="serve: offered Poisson rate, requests/s")
+    ap.add_argument("--requests", type=int, default=128, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=4,
                     help="serve: concurrent requests on the device (one stream each; the runtime has 4 hardware queues)")
     ap.add_argument("--arena-gib", type=float, default=160.0, help="serve: HBM admission budget")
@@ -521,7 +699,7 @@ def main():
         args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
                        "batch": 20, "tiles": 30}[args.workload]
 
-    import qvnt_amd
+    import qvnt_amd  # noqa: F401  (fails loudly without the HIP library)
     from qvnt_amd import QReg, circuits, op
 
     if args.workload == "tfim":
@@ -536,13 +714,15 @@ def main():
         return tiles(args)
 
     n = args.qubits
+    depth = max(40, args.warmup + args.steps)   # the depth-40 brickwork circuit
     layers = [circuits.brickwork_layer(n, l) for l in range(args.warmup + args.steps)]
     ops = [circuits.build(s, op) for s in layers]
     timed_gates = sum(len(o) for o in ops[args.warmup:])
+    label = "brickwork-%dq-fp64-depth%d, layers %d-%d timed" % (n, depth, args.warmup, args.warmup + args.steps - 1)
 
     if args.no_plain:
         fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, None, args.dense)
-        print(json.dumps({"fused": fused}), flush=True)
+        print(json.dumps({"config": {"workload": label}, "fused": fused}), flush=True)
         return
 
     reg = QReg(n)
@@ -576,8 +756,12 @@ def main():
     kernel_name = {"diag": "k_diag", "pair": "k_pair", "quad": "k_quad"}.get(dom, dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kernel_name),
+                "traffic_source": "profiles/pmc_traffic.json: rocprofv3 --pmc FETCH_SIZE x 2.000 (calibrated on "
+                                  "k_leaf, which reads the 16 GiB state once and reports 8 GiB) + WRITE_SIZE x "
+                                  "1.000 (k_init), 30-qubit FP64 brickwork layer 0",
                 "kernel": kernel_name, "launches": d["launches"], "avg_ms": round(avg_ms, 4),
-                "bytes_per_launch": per_launch_bytes}
+                "bytes_per_launch": per_launch_bytes,
+                "timing": "HIP events on the register's stream around every launch of the timed layers"}
     all_ms = sum(s["ms"] for s in stats.values())
     all_bytes = sum(s["bytes"] for s in stats.values())
 
@@ -585,31 +769,43 @@ def main():
     if not args.no_fused:
         fused = fused_leg(QReg, n, ops, args.warmup, timed_gates, value, args.dense)
 
-    cpu = None
-    c1 = None
+    host = host_info()
+    threads = cpu_threads(host)
+    cpu = c1 = refb = None
     if not args.no_cpu:
-        log("cpu baseline (oracle, host cores) ...")
-        cpu = cpu_baseline(n, args.cpu_seconds)
-        log("cpu: %.3f gates/s on %d threads" % (cpu["value"], cpu["cores"]))
-        c1 = config1_leg()
+        log("cpu baseline (oracle, %d host threads; %s) ..." % (threads, host))
+        cpu = cpu_baseline(n, threads)
+        cpu["host"] = host
+        log("cpu: %.3f gates/s on %d threads (%s)" % (cpu["value"], cpu["cores"], cpu["apply_seconds"]))
+        c1 = config1_leg(threads)
         log("20-qubit case: GPU plain %.0f gates/s, fused %.0f gates/s (max |d| vs plain %.1e); oracle %s gates/s "
             "by threads" % (c1["gpu_plain_gates_per_s"], c1["gpu_fused_gates_per_s"], c1["fused_vs_plain_max_abs_diff"],
                             c1["cpu_port_gates_per_s_by_threads"]))
+        refb = reference_bench_leg(threads)
+
+    extra = {}
+    if not args.no_extra:
+        # each leg frees its device memory before the next allocates
+        extra["tfim"] = {"f64": tfim_leg(34, "f64", 1, 50), "f32": tfim_leg(35, "f32", 1, 50)}
+        extra["tfim_pair"] = tfim_pair_leg(28, 50)
+        extra["batch"] = batch_leg(4096, 20, "f64", 0, 0, 20, 2, 64)
+        extra["served"] = serve_leg(128, 30, 4, 160.0, 3)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "gates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded brickwork circuit, angles U[0,2pi))",
-        "config": {"workload": "brickwork-%dq-fp64-depth%d-plain" % (n, args.steps), "qubits": n,
-                   "state_gib": (16 << n) / 2**30, "singleops_timed": timed_gates,
-                   "form": "plain (one in-place kernel per SingleOp)"},
+        "config": {"workload": label, "qubits": n, "state_gib": (16 << n) / 2**30, "singleops_timed": timed_gates,
+                   "form": "plain (one in-place kernel per SingleOp)", "step": "one brickwork layer (105-106 SingleOps)"},
         "roofline": roofline,
         "all_kernels_gbs": round(all_bytes / all_ms / 1e6, 1) if all_ms else None,
         "fused": fused,
         "cpu_baseline": cpu,
         "config1": c1,
+        "reference_bench": refb,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
