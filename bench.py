@@ -106,6 +106,9 @@ def cpu_baseline(n, threads, ks=(1, 4)):
     specs = circuits.brickwork_layer(n, 0)
     full = circuits.build(specs, oracle.op)
     reg = oracle.Reg(n, 0, threads=threads)
+    # untimed: the first apply also pays the first touch of the register's
+    # own buffer, which the reference's later applies do not
+    reg.apply(oracle.Op(full.gates[:1]))
     runs = []
     for k in ks:
         part = oracle.Op(full.gates[:k])
@@ -120,8 +123,9 @@ def cpu_baseline(n, threads, ks=(1, 4)):
     a = max(0.0, t0_ - b * k0)
     layer = len(full.gates)
     return {"value": done / elapsed, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": "first %s SingleOps of brickwork layer 0 at %d qubits, one MultiOp::apply each (FP64, "
-                      "out-of-place gather + per-apply copy), %.1f s" % ("/".join(str(k) for k, _ in runs), n, elapsed),
+            "sample": "first %s SingleOps of brickwork layer 0 at %d qubits, one MultiOp::apply each after one "
+                      "untimed apply (FP64, out-of-place gather + per-apply copy), %.1f s"
+                      % ("/".join(str(k) for k, _ in runs), n, elapsed),
             "apply_seconds": {str(k): round(t, 3) for k, t in runs},
             "per_singleop_s": round(b, 4), "per_apply_overhead_s": round(a, 3),
             "layer_as_one_apply_gates_per_s": round(layer / (a + layer * b), 4) if a + layer * b > 0 else None}
@@ -403,6 +407,7 @@ def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0
         span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
         o = {"circuits_per_s": round(len(res) / span, 3), "seconds": round(span, 3),
              "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+             "p90_ms": round(1e3 * float(np.percentile(lat, 90)), 2),
              "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
              "mean_ms": round(1e3 * float(lat.mean()), 2), "max_ms": round(1e3 * float(lat.max()), 2)}
         log("%s: %.2f circuits/s, p50 %.1f ms, p99 %.1f ms (%.1f s)" % (label, o["circuits_per_s"], o["p50_ms"],
@@ -596,6 +601,61 @@ def batch(args):
                             "roofline": b["roofline"]}, batch=res)), flush=True)
 
 
+PLAIN_CLASSES = [
+    # (label, constructor, args, control): every plain kernel class at the
+    # register's size, with controls and targets inside and outside one
+    # 128-byte line and the lane range (qubits 0-5)
+    ("rz q0", "rz", (0.3, 1 << 0), 0), ("rz q12", "rz", (0.3, 1 << 12), 0), ("rz top", "rz", (0.3, "top"), 0),
+    ("t q0", "t", (1 << 0,), 0), ("rzz q0q1", "rzz", (0.3, 0b11), 0),
+    ("crz t1 c0", "rz", (0.3, 1 << 1), 1 << 0), ("crz t20 c12", "rz", (0.3, 1 << 20), 1 << 12),
+    ("crz top c28", "rz", (0.3, "top"), 1 << 28), ("crz t5 top", "rz", (0.3, 1 << 5), "top"),
+    ("ry q3", "ry", (0.3, 1 << 3), 0), ("ry q12", "ry", (0.3, 1 << 12), 0), ("ry top", "ry", (0.3, "top"), 0),
+    ("x t12 c0", "x", (1 << 12,), 1 << 0), ("x top c1", "x", ("top",), 1 << 1), ("x t3 c2", "x", (1 << 3,), 1 << 2),
+    ("x top c12", "x", ("top",), 1 << 12), ("x t0 top", "x", (1 << 0,), "top"), ("x t3 c7", "x", (1 << 3,), 1 << 7),
+    ("x t12 c20", "x", (1 << 12,), 1 << 20), ("swap q0q9", "swap", ((1 << 0) | (1 << 9),), 0),
+    ("h2 q0q1", "h", (0b11,), 0), ("h2 q12q13", "h", (3 << 12,), 0), ("h2 top pair", "h", ("top2",), 0),
+    ("h2 q2q20", "h", ((1 << 2) | (1 << 20),), 0), ("h2 q5q6", "h", (3 << 5,), 0), ("ch2 q0q1 c2", "h", (0b11,), 1 << 2),
+]
+
+
+def plain_classes(args):
+    """Every plain kernel class on one register (default 30 qubits FP64):
+    --steps launches of each case, timed with HIP events on the register's
+    stream; algorithmic bytes 2 * s * 2^(n - #controls) per launch.  Ends
+    with get_absolute, whose k_leaf reads the state once (the FETCH_SIZE
+    calibration of the PMC runs; the register's k_init writes it once)."""
+    from qvnt_amd import QReg, circuits, op
+    n, prec = args.qubits, args.prec
+    top, amp = 1 << (n - 1), (16 if prec == "f64" else 8)
+
+    def fix(v):
+        return top if v == "top" else (3 << (n - 2)) if v == "top2" else v
+
+    out = {}
+    with QReg(n, prec=prec) as r:
+        r.apply(op.h(0b111))
+        for label, ctor, a, ctrl in PLAIN_CLASSES:
+            o = circuits.build([(ctor, tuple(fix(x) for x in a), fix(ctrl))], op)
+            r.apply(o)
+            r.sync()
+            r.set_timing(True)
+            r.kernel_stats(reset=True)
+            for _ in range(args.steps):
+                r.apply(o)
+            r.sync()
+            st = r.kernel_stats(reset=True)
+            r.set_timing(False)
+            k = max((c for c in st if st[c]["launches"]), key=lambda c: st[c]["ms"])
+            d = st[k]
+            gbs = d["bytes"] / d["ms"] / 1e6
+            out[label] = {"class": k, "avg_ms": round(d["ms"] / d["launches"], 4), "gbs": round(gbs, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            log("%-12s %-5s %8.4f ms %8.1f GB/s %.3f" % (label, k, d["ms"] / d["launches"], gbs, gbs / HBM_PEAK_GBS))
+        norm = r.get_absolute()
+    print(json.dumps({"workload": "plain-classes-%dq-%s" % (n, prec), "launches_per_case": args.steps,
+                      "norm": norm, "state_bytes": amp << n, "cases": out}), flush=True)
+
+
 def load_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -674,7 +734,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain form (profiling the fused kernel alone)")
     ap.add_argument("--no-extra", action="store_true",
                     help="brickwork: skip the served / TFIM / FP32-vs-FP64 / batched legs")
-    ap.add_argument("--workload", default="brickwork", choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles"])
+    ap.add_argument("--workload", default="brickwork",
+                    choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles", "plain-classes"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
     ap.add_argument("--dense", type=int, default=None,
@@ -697,7 +758,7 @@ def main():
         args.dense = 3 if args.workload == "serve" else 0
     if args.qubits is None:
         args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
-                       "batch": 20, "tiles": 30}[args.workload]
+                       "batch": 20, "tiles": 30, "plain-classes": 30}[args.workload]
 
     import qvnt_amd  # noqa: F401  (fails loudly without the HIP library)
     from qvnt_amd import QReg, circuits, op
@@ -712,6 +773,8 @@ def main():
         return batch(args)
     if args.workload == "tiles":
         return tiles(args)
+    if args.workload == "plain-classes":
+        return plain_classes(args)
 
     n = args.qubits
     depth = max(40, args.warmup + args.steps)   # the depth-40 brickwork circuit
@@ -756,9 +819,10 @@ def main():
     kernel_name = {"diag": "k_diag", "pair": "k_pair", "quad": "k_quad"}.get(dom, dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kernel_name),
-                "traffic_source": "profiles/pmc_traffic.json: rocprofv3 --pmc FETCH_SIZE x 2.000 (calibrated on "
-                                  "k_leaf, which reads the 16 GiB state once and reports 8 GiB) + WRITE_SIZE x "
-                                  "1.000 (k_init), 30-qubit FP64 brickwork layer 0",
+                "traffic_source": "profiles/pmc_traffic.json (not measured in this run): rocprofv3 --pmc FETCH_SIZE "
+                                  "x 2.000 (calibrated on k_leaf, which reads the 16 GiB state once and reports 8 GiB) "
+                                  "+ WRITE_SIZE x 1.000 (k_init), rz on qubit 12 of a 30-qubit FP64 register "
+                                  "(bench.py --workload plain-classes)",
                 "kernel": kernel_name, "launches": d["launches"], "avg_ms": round(avg_ms, 4),
                 "bytes_per_launch": per_launch_bytes,
                 "timing": "HIP events on the register's stream around every launch of the timed layers"}

@@ -80,7 +80,19 @@ struct Par {
     double m[32];        // U1 / U2 matrices
     uint32_t dagger;
     uint32_t i_pow;      // Y: !(popcount(a) + 1) as usize (atomic/y.rs:11)
+    // Index bits tested per amplitude instead of being fixed in the
+    // enumeration (bits inside one 128-byte line, so that every line is read
+    // and written whole): the gate acts where (idx & pred) == pred (its
+    // controls there) and, for the swap family, where idx & podd has odd
+    // parity; elsewhere the amplitude is written back unchanged, which is
+    // the reference's copy of psi_i[idx] (dispatch.rs:35-39).
+    uint64_t pred, podd;
 };
+
+// Whether the gate acts on the orbit of idx (see Par::pred / Par::podd).
+__device__ __forceinline__ bool acts(const Par& g, uint64_t idx) {
+    return (idx & g.pred) == g.pred && (g.podd == 0 || (dpopc(idx & g.podd) & 1u));
+}
 
 constexpr double kFrac1Sqrt2 = 0.70710678118654752440;
 
@@ -178,13 +190,18 @@ __device__ __forceinline__ cplx<R> f_pair(const Par& g, uint64_t idx, cplx<R> se
 
 // ---------------------------------------------------------- quad orbits
 // v[k] = psi[base | sel_k] with sel bits (b, a) = k; output for v-slot `s`.
+// h2.rs:22-38 for slot s, from the quad as seen from s: t0 = v[s],
+// t1 = v[s ^ 1], t2 = v[s ^ 2], t3 = v[s ^ 3].
+__device__ __forceinline__ c64 f_h2_rel(int s, c64 t0, c64 t1, c64 t2, c64 t3) {
+    if (s & 1) { t0 = cneg(t0); t2 = cneg(t2); }
+    if (s & 2) { t0 = cneg(t0); t1 = cneg(t1); }
+    return cscale(cadd(cadd(cadd(t0, t1), t2), t3), 0.5);
+}
+
 template <int K>
 __device__ __forceinline__ c64 f_quad64(const Par& g, int s, const c64* v) {
     if constexpr (K == QVNT_H2) {                                  // h2.rs:22-38
-        c64 t0 = v[s], t1 = v[s ^ 1], t2 = v[s ^ 2], t3 = v[s ^ 3];
-        if (s & 1) { t0 = cneg(t0); t2 = cneg(t2); }
-        if (s & 2) { t0 = cneg(t0); t1 = cneg(t1); }
-        return cscale(cadd(cadd(cadd(t0, t1), t2), t3), 0.5);
+        return f_h2_rel(s, v[s], v[s ^ 1], v[s ^ 2], v[s ^ 3]);
     } else {                                                       // u2.rs:22-49
         const double* m = g.m + 8 * s;
         c64 acc = cmul(c64{m[0], m[1]}, v[0]);

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, u
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-        if (in[u]) st<NT>(psi + idx[u], f_diag<K>(g, idx[u], v[u]));
+        if (in[u]) st<NT>(psi + idx[u], acts(g, idx[u]) ? f_diag<K>(g, idx[u], v[u]) : v[u]);
   }
 }
 
@@ -98,8 +98,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, u
     for (int u = 0; u < U; ++u) {
         if (in[u]) {
             const uint64_t i1 = i0[u] ^ g.a;
-            const cplx<R> o0 = f_pair<K>(g, i0[u], v0[u], v1[u]);
-            const cplx<R> o1 = f_pair<K>(g, i1, v1[u], v0[u]);
+            const bool on = acts(g, i0[u]);   // i0 and i1 differ only in the orbit mask
+            const cplx<R> o0 = on ? f_pair<K>(g, i0[u], v0[u], v1[u]) : v0[u];
+            const cplx<R> o1 = on ? f_pair<K>(g, i1, v1[u], v0[u]) : v1[u];
             st<NT>(psi + i0[u], o0);
             st<NT>(psi + i1, o1);
         }
@@ -115,11 +116,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, u
     const uint64_t base = chunk * per + threadIdx.x;
     cplx<R> v[U][4];
     uint64_t i0[U];
+    bool in[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * B;
+        const uint64_t t = (U == 2 && dep.skew) ? unit_index(chunk * B + threadIdx.x, (uint32_t)u, dep.skew)
+                                                : base + (uint64_t)u * B;
         i0[u] = deposit(t, dep);
-        if (t < groups) {
+        in[u] = t < groups;
+        if (in[u]) {
             v[u][0] = ld<NT>(psi + i0[u]);
             v[u][1] = ld<NT>(psi + (i0[u] | g.a));
             v[u][2] = ld<NT>(psi + (i0[u] | g.b));
@@ -128,11 +132,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_quad(cplx<R>* __restrict__ psi, u
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const uint64_t t = base + (uint64_t)u * B;
-        if (t < groups) {
+        if (in[u]) {
             cplx<R> o[4];
+            const bool on = acts(g, i0[u]);
 #pragma unroll
-            for (int s = 0; s < 4; ++s) o[s] = f_quad<K>(g, s, v[u]);
+            for (int s = 0; s < 4; ++s) o[s] = on ? f_quad<K>(g, s, v[u]) : v[u][s];
             st<NT>(psi + i0[u], o[0]);
             st<NT>(psi + (i0[u] | g.a), o[1]);
             st<NT>(psi + (i0[u] | g.b), o[2]);
@@ -173,7 +177,90 @@ __global__ __launch_bounds__(kMaxBlock) void k_pair_lanes(cplx<R>* __restrict__ 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const cplx<R> other{__shfl_xor(v[u].re, lm), __shfl_xor(v[u].im, lm)};
-            if (in[u]) st<true>(psi + idx[u], f_pair<K>(g, idx[u], v[u], other));
+            if (in[u]) st<true>(psi + idx[u], acts(g, idx[u]) ? f_pair<K>(g, idx[u], v[u], other) : v[u]);
+        }
+    }
+}
+
+// H2 gates whose two bits lie in the lane bits (qubits 0..5, no deposited
+// control among them): each lane loads one amplitude and takes the other
+// three of its quad from lanes ^ a, ^ b, ^ (a|b); the output is the
+// reference's formula for its slot (h2.rs:22-38, f_h2_rel), bit for bit.
+template <class R, int U>
+__global__ __launch_bounds__(kMaxBlock) void k_h2_lanes(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
+    const uint64_t B = blockDim.x, per = B * U;
+    const uint64_t nchunks = (groups + per - 1) / per;
+    const int la = (int)g.a, lb = (int)g.b;
+    const int s = ((threadIdx.x & la) ? 1 : 0) | ((threadIdx.x & lb) ? 2 : 0);   // this lane's slot
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint64_t base = chunk * per + threadIdx.x;
+        cplx<R> v[U];
+        uint64_t idx[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t t = (U == 4 && dep.skew) ? unit_index(chunk * B + threadIdx.x, (uint32_t)u, dep.skew, 2)
+                                                    : base + (uint64_t)u * B;
+            idx[u] = deposit(t, dep);
+            in[u] = t < groups;
+            v[u] = cplx<R>{(R)0, (R)0};
+            if (in[u]) v[u] = ld<true>(psi + idx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const cplx<R> xa{__shfl_xor(v[u].re, la), __shfl_xor(v[u].im, la)};
+            const cplx<R> xb{__shfl_xor(v[u].re, lb), __shfl_xor(v[u].im, lb)};
+            const cplx<R> xab{__shfl_xor(v[u].re, la | lb), __shfl_xor(v[u].im, la | lb)};
+            const cplx<R> o = down<R>(f_h2_rel(s, up(v[u]), up(xa), up(xb), up(xab)));
+            if (in[u]) st<true>(psi + idx[u], acts(g, idx[u]) ? o : v[u]);
+        }
+    }
+}
+
+// H2 gates with one bit in the lane bits (qubits 0..5) and the other,
+// m, above them: each lane loads amplitudes idx and idx | m (two contiguous
+// runs per wave) and takes the lane partner of each from lane ^ l; the
+// output is the reference's formula for its slot (f_h2_rel), bit for bit.
+template <class R, int U>
+__global__ __launch_bounds__(kMaxBlock) void k_h2_half(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
+    const uint64_t B = blockDim.x, per = B * U;
+    const uint64_t nchunks = (groups + per - 1) / per;
+    const bool lane_is_a = g.a < 64;                       // which of a, b lies in the lane bits
+    const int l = (int)(lane_is_a ? g.a : g.b);
+    const uint64_t m = lane_is_a ? g.b : g.a;
+    const int sl = (threadIdx.x & l) ? 1 : 0;              // this lane's bit l
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const uint64_t base = chunk * per + threadIdx.x;
+        cplx<R> v0[U], v1[U];
+        uint64_t idx[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t t = (U == 2 && dep.skew) ? unit_index(chunk * B + threadIdx.x, (uint32_t)u, dep.skew)
+                                                    : base + (uint64_t)u * B;
+            idx[u] = deposit(t, dep);                      // bit m is 0
+            in[u] = t < groups;
+            v0[u] = v1[u] = cplx<R>{(R)0, (R)0};
+            if (in[u]) {
+                v0[u] = ld<true>(psi + idx[u]);
+                v1[u] = ld<true>(psi + (idx[u] | m));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const c64 p0 = up(v0[u]), p1 = up(v1[u]);
+            const c64 q0 = up(cplx<R>{__shfl_xor(v0[u].re, l), __shfl_xor(v0[u].im, l)});
+            const c64 q1 = up(cplx<R>{__shfl_xor(v1[u].re, l), __shfl_xor(v1[u].im, l)});
+            // slots: bit 0 = a, bit 1 = b; from idx (bit m = 0) and from idx | m
+            const int s0 = lane_is_a ? sl : (sl << 1), s1 = lane_is_a ? (sl | 2) : ((sl << 1) | 1);
+            // t1 = value at ^a, t2 = at ^b, t3 = at ^ab
+            const c64 o0 = lane_is_a ? f_h2_rel(s0, p0, q0, p1, q1) : f_h2_rel(s0, p0, p1, q0, q1);
+            const c64 o1 = lane_is_a ? f_h2_rel(s1, p1, q1, p0, q0) : f_h2_rel(s1, p1, p0, q1, q0);
+            if (in[u]) {
+                const bool on = acts(g, idx[u]);
+                st<true>(psi + idx[u], on ? down<R>(o0) : v0[u]);
+                st<true>(psi + (idx[u] | m), on ? down<R>(o1) : v1[u]);
+            }
         }
     }
 }
@@ -197,8 +284,9 @@ bool lane_setting() {
     return on;
 }
 // Orbits per thread: pair kernels two (placed 128 KiB apart, see
-// skew_setting), quad kernels one; QVNT_UNROLL_PAIR / QVNT_UNROLL_QUAD
-// (1, 2 or 4) override.
+// skew_setting), quad kernels one (measured at 30 qubits: two skewed quads
+// were 3 % faster on qubits 28-29 but 3-9 % slower on 12-13 and 2-20);
+// QVNT_UNROLL_PAIR / QVNT_UNROLL_QUAD (1, 2 or 4) override.
 int units_from_env(const char* name, int def) {
     const char* s = std::getenv(name);
     const int v = s ? std::atoi(s) : def;
@@ -223,8 +311,9 @@ int unroll_setting() {
 // above kSkewMaxBytes one stream is as fast or faster (34 qubits: 6.13
 // against 5.62 TB/s), so diagonals of larger states keep one; k_pair gains
 // at 30 qubits (5.71 -> 6.04 TB/s) and loses nothing at 34.
-// QVNT_DIAG_SKEW / QVNT_PAIR_SKEW override S at every size (0: the units
-// blockDim apart).
+// Quad kernels place their two orbits the same way as pairs.
+// QVNT_DIAG_SKEW / QVNT_PAIR_SKEW / QVNT_QUAD_SKEW override S at every size
+// (0: the units blockDim apart).
 constexpr uint64_t kSkewMaxBytes = 32ull << 30;
 uint32_t skew_setting(GClass cls, size_t amp_bytes, uint64_t state_bytes) {
     static const int diag_env = [] {
@@ -235,9 +324,12 @@ uint32_t skew_setting(GClass cls, size_t amp_bytes, uint64_t state_bytes) {
         const char* s = std::getenv("QVNT_PAIR_SKEW");
         return s ? std::atoi(s) : -1;
     }();
-    const int env = cls == GClass::Diag ? diag_env : cls == GClass::Pair ? pair_env : 0;
+    static const int quad_env = [] {
+        const char* s = std::getenv("QVNT_QUAD_SKEW");
+        return s ? std::atoi(s) : -1;
+    }();
+    const int env = cls == GClass::Diag ? diag_env : cls == GClass::Pair ? pair_env : quad_env;
     if (env >= 0) return env <= 40 ? (uint32_t)env : 0u;
-    if (cls != GClass::Diag && cls != GClass::Pair) return 0u;
     if (cls == GClass::Diag && state_bytes > kSkewMaxBytes) return 0u;
     return amp_bytes == 16 ? 13u : 14u;  // 128 KiB
 }
@@ -272,9 +364,12 @@ void fix_bits(Deposit& d, uint64_t bits, bool one) {
     if (one) d.set |= bits;
 }
 
-// Gate parameters in FP64 for both amplitude precisions.
-Par make_par(const qvnt_gate& g) {
+// Gate parameters in FP64 for both amplitude precisions; pred / podd: the
+// index bits tested per amplitude (Par).
+Par make_par(const qvnt_gate& g, uint64_t pred, uint64_t podd) {
     Par p{};
+    p.pred = pred;
+    p.podd = podd;
     p.a = g.a;
     p.b = g.b;
     p.re = g.re;
@@ -286,11 +381,10 @@ Par make_par(const qvnt_gate& g) {
 }
 
 template <class R, int U, bool NT>
-int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d0) {
+int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d0, const Par& p) {
     Deposit d = d0;
     const uint32_t S = U == 2 ? skew_setting(cls, sizeof(cplx<R>), r->len * sizeof(cplx<R>)) : 0u;
     d.skew = (S && groups >= (2ull << S)) ? S : 0u;
-    const Par p = make_par(gate);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
     const unsigned B = block_setting();
     const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
@@ -339,16 +433,15 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
     return QVNT_OK;
 }
 
-// Non-temporal accesses when every amplitude's 128-byte line is touched by
-// one load and one store only: always for diagonals; for pairs and quads
-// when the partner offsets are at least one line (8 FP64 or 16 FP32
-// amplitudes: the partners lie in other lines, read by other instructions).
+// Non-temporal accesses when every 128-byte line is read by one load
+// instruction and written by one store (launch_unrolled); the lane kernels
+// always qualify.
 template <class R>
-int launch_lanes(qvnt_reg* r, const qvnt_gate& gate) {
+int launch_lanes(qvnt_reg* r, const qvnt_gate& gate, uint64_t pred) {
     Deposit d{};
-    fix_bits(d, gate.ctrl, true);
+    fix_bits(d, gate.ctrl & ~pred, true);
     const uint64_t groups = r->len >> d.n;
-    const Par p = make_par(gate);
+    const Par p = make_par(gate, pred, 0);
     cplx<R>* psi = static_cast<cplx<R>*>(r->psi);
     const unsigned B = block_setting();
     constexpr int U = 4;  // four amplitudes per lane: as many loads in flight as k_pair's two orbits
@@ -379,16 +472,53 @@ int launch_lanes(qvnt_reg* r, const qvnt_gate& gate) {
 }
 
 template <class R>
-int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d) {
+int launch_h2_lanes(qvnt_reg* r, const qvnt_gate& gate, uint64_t pred) {
+    Deposit d{};
+    fix_bits(d, gate.ctrl & ~pred, true);
+    const uint64_t groups = r->len >> d.n;
+    const Par p = make_par(gate, pred, 0);
+    const unsigned B = block_setting();
+    constexpr int U = 4;
+    const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+    const uint32_t S = lanes_skew_setting(sizeof(cplx<R>), r->len * sizeof(cplx<R>));
+    d.skew = (S >= 6 && groups >= (4ull << S)) ? S : 0u;
+    const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid)), block(B);
+    hipLaunchKernelGGL((k_h2_lanes<R, U>), grid, block, 0, r->stream, static_cast<cplx<R>*>(r->psi), groups, d, p);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+template <class R>
+int launch_h2_half(qvnt_reg* r, const qvnt_gate& gate, uint64_t pred) {
+    const uint64_t m = gate.a < 64 ? gate.b : gate.a;
+    Deposit d{};
+    fix_bits(d, m, false);
+    fix_bits(d, gate.ctrl & ~pred, true);
+    const uint64_t groups = r->len >> d.n;
+    constexpr int U = 2;
+    const uint32_t S = skew_setting(GClass::Pair, sizeof(cplx<R>), r->len * sizeof(cplx<R>));
+    d.skew = (S >= 6 && groups >= (2ull << S)) ? S : 0u;
+    const Par p = make_par(gate, pred, 0);
+    const unsigned B = block_setting();
+    const uint64_t nchunks = (groups + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+    const dim3 grid((unsigned)(nchunks < kMaxGrid ? nchunks : kMaxGrid)), block(B);
+    hipLaunchKernelGGL((k_h2_half<R, U>), grid, block, 0, r->stream, static_cast<cplx<R>*>(r->psi), groups, d, p);
+    QVNT_HIP(hipGetLastError());
+    return QVNT_OK;
+}
+
+template <class R>
+int launch_unrolled(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t groups, const Deposit& d, const Par& p) {
+    // every load and store instruction covers whole 128-byte lines exactly
+    // when no index bit inside a line is fixed in the enumeration
     constexpr uint64_t line_amps = 128 / (2 * sizeof(R));
-    const uint64_t partner = cls == GClass::Diag ? 0 : cls == GClass::Pair ? gate.a : (gate.a | gate.b);
-    const bool nt = cls == GClass::Diag || (partner & (line_amps - 1)) == 0;
+    const bool nt = (d.zeros & (line_amps - 1)) == 0;
     const int us = cls == GClass::Diag ? unroll_setting() : pair_unroll_setting(cls);
     if (us == 1)
-        return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d) : launch_kernel<R, 1, false>(r, gate, cls, groups, d);
+        return nt ? launch_kernel<R, 1, true>(r, gate, cls, groups, d, p) : launch_kernel<R, 1, false>(r, gate, cls, groups, d, p);
     if (us == 4)
-        return nt ? launch_kernel<R, 4, true>(r, gate, cls, groups, d) : launch_kernel<R, 4, false>(r, gate, cls, groups, d);
-    return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d) : launch_kernel<R, 2, false>(r, gate, cls, groups, d);
+        return nt ? launch_kernel<R, 4, true>(r, gate, cls, groups, d, p) : launch_kernel<R, 4, false>(r, gate, cls, groups, d, p);
+    return nt ? launch_kernel<R, 2, true>(r, gate, cls, groups, d, p) : launch_kernel<R, 2, false>(r, gate, cls, groups, d, p);
 }
 
 }  // namespace
@@ -438,6 +568,14 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
     const bool diag = is_diagonal_kind(k);
     const bool quad = (k == QVNT_H2 || k == QVNT_U2);
 
+    // Index bits inside one 128-byte line (8 FP64 or 16 FP32 amplitudes) are
+    // never fixed in the enumeration: fixing one (a control, or the |1> half
+    // of a one-qubit phase gate) would touch half of every line, and lines
+    // written in part cost a read-modify-write in the memory system.  They
+    // are tested per amplitude instead (Par::pred, Par::podd).
+    const uint64_t line_mask = (r->prec == QVNT_F64 ? 8u : 16u) - 1u;
+    const uint64_t pred = gate.ctrl & line_mask & lmask;
+    uint64_t podd = 0;
     Deposit d{};
     GClass cls;
     const uint64_t a_in = gate.a & lmask;
@@ -446,7 +584,9 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
         const bool single_bit = popc(a_in) == 1;
         const bool zero_fixed = (k == QVNT_Z || k == QVNT_S || k == QVNT_T || k == QVNT_PHI);
         if (zero_fixed && a_in == 0) return QVNT_OK;               // identity
-        if (zero_fixed && single_bit) fix_bits(d, a_in, true);     // the |0> half is unchanged
+        // the |0> half is unchanged: skip it, unless the bit is inside a
+        // line (the formula leaves it unchanged there)
+        if (zero_fixed && single_bit && !(a_in & line_mask)) fix_bits(d, a_in, true);
     } else if (quad) {
         cls = GClass::Quad;
         fix_bits(d, gate.a | gate.b, false);
@@ -455,10 +595,14 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
         if (gate.a == 0) return QVNT_OK;                           // X0 / Y0: identity
         const uint64_t hi = 1ull << msb(gate.a);
         fix_bits(d, hi, false);
-        if (k == QVNT_SWAP || k == QVNT_ISWAP || k == QVNT_SQRT_SWAP || k == QVNT_SQRT_ISWAP)
-            fix_bits(d, gate.a & ~hi, true);                       // odd-parity orbits only
+        if (k == QVNT_SWAP || k == QVNT_ISWAP || k == QVNT_SQRT_SWAP || k == QVNT_SQRT_ISWAP) {
+            // odd-parity orbits only (even ones are unchanged)
+            const uint64_t lo = gate.a & ~hi;
+            if (lo & line_mask) podd = gate.a;
+            else fix_bits(d, lo, true);
+        }
     }
-    fix_bits(d, gate.ctrl, true);
+    fix_bits(d, gate.ctrl & ~pred, true);
     const uint64_t groups = r->len >> d.n;
 
     const double bytes = 2.0 * (double)r->amp_bytes() * (double)(r->len >> popc(gate.ctrl));
@@ -466,13 +610,26 @@ int launch_gate(qvnt_reg* r, const qvnt_gate& gate) {
     Timed t(r, kc, bytes);
     // partners within a wave's 64 consecutive amplitudes: lane exchange
     const bool swap_family = k == QVNT_SWAP || k == QVNT_ISWAP || k == QVNT_SQRT_SWAP || k == QVNT_SQRT_ISWAP;
-    if (cls == GClass::Pair && !swap_family && (a_in & ~63ull) == 0 && (gate.ctrl & 63ull) == 0 &&
-        r->ns >= 6 + popc(gate.ctrl & lmask) && lane_setting()) {
-        if (r->prec == QVNT_F64) return launch_lanes<double>(r, gate);
-        return launch_lanes<float>(r, gate);
+    const uint64_t hi_ctrl = gate.ctrl & ~pred;
+    if (cls == GClass::Pair && !swap_family && (a_in & ~63ull) == 0 && (hi_ctrl & 63ull) == 0 &&
+        r->ns >= 6 + popc(hi_ctrl & lmask) && lane_setting()) {
+        if (r->prec == QVNT_F64) return launch_lanes<double>(r, gate, pred);
+        return launch_lanes<float>(r, gate, pred);
     }
-    if (r->prec == QVNT_F64) return launch_unrolled<double>(r, gate, cls, groups, d);
-    return launch_unrolled<float>(r, gate, cls, groups, d);
+    const bool a_lane = gate.a < 64, b_lane = gate.b < 64;
+    if (cls == GClass::Quad && k == QVNT_H2 && (a_lane || b_lane) && (hi_ctrl & 63ull) == 0 && lane_setting()) {
+        if (a_lane && b_lane && r->ns >= 6 + popc(hi_ctrl & lmask)) {
+            if (r->prec == QVNT_F64) return launch_h2_lanes<double>(r, gate, pred);
+            return launch_h2_lanes<float>(r, gate, pred);
+        }
+        if (!(a_lane && b_lane) && r->ns >= 7 + popc(hi_ctrl & lmask)) {
+            if (r->prec == QVNT_F64) return launch_h2_half<double>(r, gate, pred);
+            return launch_h2_half<float>(r, gate, pred);
+        }
+    }
+    const Par p = make_par(gate, pred, podd);
+    if (r->prec == QVNT_F64) return launch_unrolled<double>(r, gate, cls, groups, d, p);
+    return launch_unrolled<float>(r, gate, cls, groups, d, p);
 }
 
 int launch_init(qvnt_reg* r, uint64_t state) {

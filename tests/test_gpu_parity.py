@@ -404,3 +404,71 @@ def test_u2_controlled_on_its_b_qubit_is_refused_by_apply():
             r.apply(cu)
         assert e.value.code == E_CTRL_OVERLAP
         assert np.array_equal(r.get_state(), before)
+
+
+LINE_CASES = [  # gates with qubits or controls inside one 128-byte line (FP64: qubits 0-2, FP32: 0-3)
+    ("x", (1 << 12,), 1 << 0), ("x", (1 << 3,), 1 << 2), ("x", (1 << 1,), 1 << 0), ("x", (1 << 9,), (1 << 1) | (1 << 11)),
+    ("rz", (0.7, 1 << 1), 1 << 0), ("rz", (0.7, 1 << 10), 1 << 3), ("ry", (0.4, 1 << 2), 1 << 3),
+    ("t", (1 << 0,), 0), ("s", (1 << 2,), 1 << 5), ("z", (1 << 3,), 0), ("phi", (0.3, 1 << 1), 0),
+    ("swap", ((1 << 0) | (1 << 9),), 0), ("i_swap", ((1 << 1) | (1 << 2),), 1 << 8),
+    ("sqrt_swap", ((1 << 3) | (1 << 4),), 1 << 0), ("sqrt_i_swap", ((1 << 2) | (1 << 12),), 0),
+    ("h", (0b11,), 0), ("h", (0b110000,), 1 << 1), ("h", ((1 << 2) | (1 << 9),), 1 << 0), ("h", (0b1100,), 0),
+    ("rxx", (0.9, 0b101), 1 << 1), ("ryy", (0.9, (1 << 3) | (1 << 7)), 1 << 2), ("rzz", (0.2, 0b11), 1 << 2),
+    ("y", (1 << 4,), 1 << 0), ("u3", (0.1, 0.2, 0.3, 1 << 5), 1 << 1),
+]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_gates_inside_a_cache_line(orc, prec):
+    """Controls and one-qubit phase bits inside a 128-byte line are tested per
+    amplitude rather than skipped (so every line is written whole): each
+    SingleOp on a random state equals the oracle -- bit for bit in FP64, and
+    in FP32 the oracle's FP64 result on the widened input, rounded once."""
+    g = np.random.default_rng(5150)
+    n = 13
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    dt = np.complex128 if prec == "f64" else np.complex64
+    psi = psi.astype(dt)
+    ref = orc.Reg(n)
+    with QReg(n, prec=prec) as r:
+        r.set_state(psi)
+        for ctor, args, ctrl in LINE_CASES:
+            o_dev = circuits.build([(ctor, args, ctrl)], op)
+            o_ref = circuits.build([(ctor, args, ctrl)], orc.op)
+            for gd, gc in zip(o_dev.gates(), o_ref.gates):
+                ref.set_state(psi.astype(np.complex128))
+                ref.apply(orc.Op([gc]))
+                want = ref.state().astype(dt)
+                r.apply(qvnt_amd.from_gates([gd]))
+                got = r.get_state()
+                assert np.array_equal(got, want), (ctor, args, ctrl, float(np.max(np.abs(got - want))))
+                psi = got
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_h2_with_one_bit_in_the_lanes(orc, prec):
+    """H2 with one bit in qubits 0-5 and the other above (the half-lane
+    kernel), in both orientations of a_mask / b_mask, with and without a
+    control inside a line: bit for bit against the oracle (FP32: the FP64
+    result on the widened input, rounded once)."""
+    g = np.random.default_rng(6160)
+    n = 12
+    psi = g.normal(size=1 << n) + 1j * g.normal(size=1 << n)
+    dt = np.complex128 if prec == "f64" else np.complex64
+    psi = (psi / np.linalg.norm(psi)).astype(dt)
+    ref = orc.Reg(n)
+    with QReg(n, prec=prec) as r:
+        r.set_state(psi)
+        for a, b, ctrl in [(1 << 2, 1 << 9, 0), (1 << 9, 1 << 2, 0), (1 << 1, 1 << 7, 1 << 0), (1 << 11, 1 << 5, 1 << 2),
+                           (1 << 0, 1 << 6, 1 << 8)]:
+            gd, gc = qvnt_amd.Gate(), orc.Gate()
+            for x in (gd, gc):
+                x.kind, x.act, x.a, x.b, x.ctrl = qvnt_amd.KIND["H2"], a | b, a, b, ctrl
+            ref.set_state(psi.astype(np.complex128))
+            ref.apply(orc.Op([gc]))
+            want = ref.state().astype(dt)
+            r.apply(qvnt_amd.from_gates([gd]))
+            got = r.get_state()
+            assert np.array_equal(got, want), (a, b, ctrl, float(np.max(np.abs(got - want))))
+            psi = got
