@@ -12,7 +12,8 @@
 // the queue, up to kMaxBypass times -- and enqueues its whole life on the
 // slot's stream: init, apply (fused or plain), measurement tree and draws,
 // outcome copy, event.  It then waits for the event, returns the arena
-// block and posts the result.  Workers enqueue in parallel, so a long
+// block and posts the result.  Requests of 2^27 amplitudes or more run one
+// at a time (kBigQubits); smaller ones overlap with them and each other.  Workers enqueue in parallel, so a long
 // request's thousands of launches do not hold up admission into the other
 // slots, and requests on different slots overlap on the device.  Kernels
 // are deterministic (fixed reduction orders, no atomics), so a request's
@@ -37,6 +38,11 @@ namespace {
 
 constexpr size_t kAlign = 256;
 constexpr int kMaxBypass = 8;
+// Requests of at least 2^kBigQubits amplitudes run one at a time: each of
+// their passes has tens of thousands of tiles, far more than the GPU holds
+// at once, so two of them side by side only stretch each other (and the
+// tail latency), while small requests still overlap with them.
+constexpr unsigned kBigQubits = 27;
 size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
 // First-fit sub-allocator over one device allocation.
@@ -105,6 +111,7 @@ struct Record {
     uint64_t len = 0;
     size_t bytes = 0;  // arena bytes: state, tree, draws, outcomes, total, programs
     size_t prog_bytes = 0;
+    bool big = false;  // q_num >= kBigQubits
     int bypassed = 0;
     qvnt_result res{};
     size_t arena_off = 0;
@@ -133,6 +140,7 @@ struct qvnt_server {
     std::deque<Record*> pending;
     std::deque<Record*> done;
     std::unordered_set<uint64_t> ids;  // ids submitted and not yet returned by poll
+    int big_running = 0;               // admitted requests of >= kBigQubits
     std::vector<std::thread> workers;  // one per slot
     bool stop = false;
     int device = 0;
@@ -241,6 +249,7 @@ void qvnt_server::finish(Slot& sl, int status) {
         for (uint64_t k = 0; k < q.shots; ++k) q.outcomes[k] = o[k] & mask;
     }
     arena.release(rec->arena_off, rec->bytes);
+    if (rec->big) --big_running;
     sl.rec = nullptr;
     done.push_back(rec);
 }
@@ -252,6 +261,7 @@ Record* qvnt_server::admit() {
     for (auto it = pending.begin(); it != pending.end(); ++it) {
         Record* rec = *it;
         size_t off = 0;
+        if (rec->big && big_running > 0) continue;   // waits for the running large request
         if (!arena.alloc(rec->bytes, &off)) {
             if (it == pending.begin() && rec->bypassed >= kMaxBypass) return nullptr;
             continue;
@@ -259,6 +269,7 @@ Record* qvnt_server::admit() {
         if (it != pending.begin()) pending.front()->bypassed++;
         rec->arena_off = off;
         rec->res.t_admit = now();
+        if (rec->big) ++big_running;
         pending.erase(it);
         return rec;
     }
@@ -336,6 +347,7 @@ int qvnt_server_submit(qvnt_server* s, const qvnt_request* q) {
     if (!rec) return fail(QVNT_E_OOM, "host allocation failed");
     rec->req = *q;
     rec->len = 1ull << (q->q_num < 3 ? 3 : q->q_num);
+    rec->big = q->q_num >= kBigQubits;
     rec->gates = q->op->g;
     for (const qvnt_gate& g : rec->gates) {
         bool noop = false;

@@ -120,3 +120,30 @@ def test_duplicate_live_id_is_refused():
         s.submit(5, 3, o, 0b111, draws)
         res = s.drain(1)
         assert res[0]["status"] == 0
+
+
+def test_large_requests_run_one_at_a_time():
+    """Requests of 2^27 amplitudes or more are admitted one at a time (their
+    [admit, done] intervals on the server clock do not overlap), while small
+    requests run beside them; outcomes equal those served alone."""
+    g = np.random.default_rng(77)
+    reqs = []
+    for k, n in enumerate([27, 10, 27, 12, 11, 27, 9]):
+        specs = circuits.random_mixed(n, 30, g)
+        mask = int(sum(1 << int(q) for q in g.choice(n, 4, replace=False)))
+        draws = np.array([qvnt_amd.uniform(90, k, j) for j in range(64)])
+        reqs.append((n, specs, mask, draws))
+    ops = [circuits.build(specs, op) for _, specs, _, _ in reqs]
+    with Server(16 << 30, slots=4) as s:
+        for k, (n, _, mask, draws) in enumerate(reqs):
+            s.submit(k, n, ops[k], mask, draws)
+        mix = {r["id"]: r for r in s.drain(len(reqs))}
+    assert len(mix) == len(reqs) and all(r["status"] == 0 for r in mix.values())
+    big = sorted((mix[k]["t_admit"], mix[k]["t_done"]) for k, (n, _, _, _) in enumerate(reqs) if n >= 27)
+    for (a0, d0), (a1, d1) in zip(big, big[1:]):
+        assert a1 >= d0, big
+    with Server(16 << 30, slots=1) as s:
+        for k, (n, _, mask, draws) in enumerate(reqs):
+            s.submit(k, n, ops[k], mask, draws)
+            r = s.drain(1)[0]
+            assert np.array_equal(r["outcomes"], mix[k]["outcomes"]) and r["total"] == mix[k]["total"]
