@@ -207,3 +207,56 @@ def test_fused_30q_round_trip():
         outs = r.sample_mask((1 << n) - 1, [qvnt_amd.uniform(1, 2, k) for k in range(16)])
         say("passes", p1, p2)
         assert all(int(o) == start for o in outs)
+
+
+def _inverse(specs):
+    """The exact inverse of brickwork / TFIM / mixed specs: reversed order,
+    rotations negated (a rotation's .dgr() is -U in the reference)."""
+    inv = []
+    for ctor, args, ctrl in reversed(specs):
+        if ctor == "u3":
+            th, ph, la, m = args
+            inv += [("rz", (-ph, m), ctrl), ("ry", (-th, m), ctrl), ("rz", (-la, m), ctrl)]
+        elif ctor in ("rx", "ry", "rz", "rxx", "ryy", "rzz", "phi"):
+            inv.append((ctor, (-args[0],) + tuple(args[1:]), ctrl))
+        else:
+            inv.append((ctor, args, ctrl))
+    return inv
+
+
+@pytest.mark.parametrize("workload", ["brickwork", "mixed"])
+def test_fused_equals_plain_at_30_qubits(workload):
+    """At the full 30 qubits (16 GiB per register) the fused form agrees with
+    the plain form -- which equals the oracle bit for bit -- to 1e-12,
+    compared on the device: 3 brickwork layers, or 300 gates of the
+    batched/serving gate set (swaps, iswaps, controlled x, rxx/ryy/rzz)."""
+    n = 30
+    say("fused vs plain at 30 q: %s" % workload)
+    if workload == "brickwork":
+        specs = sum((circuits.brickwork_layer(n, l) for l in range(3)), [])
+    else:
+        specs = circuits.random_mixed(n, 300, np.random.default_rng(3030))
+    o = circuits.build(specs, op)
+    with QReg(n) as a, QReg(n) as b:
+        a.apply(o)
+        b.apply_fused(o)
+        err, infid, _, na, nb = a.compare(b)
+    say("  max |d| %.2e, infidelity %.2e" % (err, infid))
+    assert err <= TOL and abs(infid) <= 1e-12 and abs(na - 1.0) <= 1e-12 and abs(nb - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("n,prec", [(34, "f64"), (35, "f32")])
+def test_tfim_ring_round_trip_largest_register(n, prec):
+    """The TFIM ring at the configs' full sizes (256 GiB): 5 Trotter steps
+    through the fused form and then their exact inverse return |0...0>
+    (every shot 0); the norm holds to 1e-12 (FP64) / 1e-5 (FP32)."""
+    say("tfim round trip n=%d %s" % (n, prec))
+    fwd = circuits.tfim_step(n) * 5
+    with QReg(n, prec=prec) as r:
+        r.apply_fused(circuits.build(fwd, op))
+        tol = 1e-12 if prec == "f64" else 1e-5
+        assert abs(r.get_absolute() - 1.0) < tol
+        r.apply_fused(circuits.build(_inverse(fwd), op))
+        outs = r.sample_mask((1 << n) - 1, [qvnt_amd.uniform(8, n, k) for k in range(8)])
+        assert all(int(x) == 0 for x in outs)
+        assert abs(r.get_absolute() - 1.0) < tol

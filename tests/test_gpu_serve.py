@@ -147,3 +147,34 @@ def test_large_requests_run_one_at_a_time():
             s.submit(k, n, ops[k], mask, draws)
             r = s.drain(1)[0]
             assert np.array_equal(r["outcomes"], mix[k]["outcomes"]) and r["total"] == mix[k]["total"]
+
+
+def test_serving_sizes_alone_equals_mix():
+    """Requests at the serving workload's sizes (log-uniform 12-30 qubits,
+    two of them 28 and 30 qubits) with 100-300 gates of the mixed set and
+    1024 shots: outcomes and totals bit for bit the same served one at a
+    time and all at once."""
+    reqs = circuits.serving_requests(22, seed=2024, n_lo=12, n_hi=26, g_lo=100, g_hi=300)
+    g = np.random.default_rng(2025)
+    for n in (28, 30):
+        specs = circuits.random_mixed(n, 150, g)
+        mask = int(sum(1 << int(q) for q in g.choice(n, 8, replace=False)))
+        reqs.append((n, specs, mask, np.array([qvnt_amd.uniform(2026, n, k) for k in range(1024)])))
+    ops = [circuits.build(specs, op) for _, specs, _, _ in reqs]
+    print("serving sizes:", sorted(n for n, _, _, _ in reqs), flush=True)
+    with Server(48 << 30, slots=1) as s1:
+        alone = {}
+        for k, (n, _, mask, draws) in enumerate(reqs):
+            s1.submit(k, n, ops[k], mask, draws, dense_min=3)
+            r = s1.drain(1)[0]
+            alone[r["id"]] = r
+    with Server(48 << 30, slots=4) as s4:
+        for k, (n, _, mask, draws) in enumerate(reqs):
+            s4.submit(k, n, ops[k], mask, draws, dense_min=3)
+        mix = {r["id"]: r for r in s4.drain(len(reqs))}
+    for k, (n, _, mask, _) in enumerate(reqs):
+        a, m = alone[k], mix[k]
+        assert a["status"] == 0 and m["status"] == 0
+        assert np.array_equal(a["outcomes"], m["outcomes"]) and a["total"] == m["total"]
+        assert abs(a["total"] - 1.0) < 1e-12
+        assert np.all((a["outcomes"] & ~np.uint64(mask)) == 0)
