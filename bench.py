@@ -25,8 +25,9 @@ its device memory before the next:
              50 steps through the fused form as one call;
   tfim_pair  28 qubits, FP32 against FP64 after every one of 50 steps
              (fused and plain): max |dpsi| and infidelity;
-  batch      4096 registers of 10-20 qubits, FP64, autotuned and run as one
-             hipGraph: gates/s, ms per run, HBM and matrix-core rates;
+  batch      4096 registers of 10-20 qubits, FP64 and FP32, autotuned and
+             run as one hipGraph: gates/s, ms per run, HBM and matrix-core
+             rates (the plain register-by-register loop beside FP64);
   served     128 serving requests (12-30 qubits, 100-2000 gates, 1024
              shots): one-at-a-time plain and fused loops, the
              continuous-batching server at saturation, and open-loop
@@ -528,15 +529,28 @@ def batch_leg(registers, n_hi, prec, tile_cap, dense, steps, warmup, check, plai
             info["dense_phases"], info["phases"], info["tile_cap"], info["dense_min"]))
         b.run(graph=True)
         b.sync()
-        errs = []
+        errs, first = [], []
         for k in range(sample):
             got = b.get_state(k)
+            first.append(got)
             if plain_states:
                 errs.append(float(np.max(np.abs(got.astype(np.complex128) - plain_states[k]))))
         for _ in range(warmup):
             b.run(graph=True)
         ms_graph = b.time(steps, graph=True)
         ms_eager = b.time(max(1, steps // 2), graph=False)
+        persist = None
+        if info["persist_blocks"]:
+            # the persistent form: one cooperative launch over all rounds
+            b.reset()
+            b.run(graph="persistent")
+            b.sync()
+            same = all(np.array_equal(b.get_state(k), first[k]) for k in range(sample))
+            ms_persist = b.time(steps, graph="persistent")
+            persist = {"ms_per_run": round(ms_persist, 3), "blocks": info["persist_blocks"],
+                       "value": round(single / (ms_persist * 1e-3), 1), "equal_to_rounds": same}
+            log("persistent launch (%d blocks): %.2f ms/run; equal to the rounds on %d registers: %s" % (
+                info["persist_blocks"], ms_persist, sample, same))
     gbs = info["bytes"] / (ms_graph * 1e-3) / 1e9
     value = single / (ms_graph * 1e-3)
     tflops = info["dense_flops"] / (ms_graph * 1e-3) / 1e12
@@ -555,6 +569,7 @@ def batch_leg(registers, n_hi, prec, tile_cap, dense, steps, warmup, check, plai
                       "mfma": {"achieved": round(tflops, 3), "peak": MFMA_PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                                "frac": round(tflops / MFMA_PEAK_TFLOPS[prec], 4), "flops_per_run": info["dense_flops"],
                                "note": "dense 16x16 phases (real 32x32 GEMMs) only"}}
+    out["batched"]["persistent"] = persist
     if "plain" in out:
         out["batched"]["speedup_vs_plain"] = round(value / out["plain"]["value"], 2)
     return out
@@ -852,7 +867,8 @@ def main():
         # each leg frees its device memory before the next allocates
         extra["tfim"] = {"f64": tfim_leg(34, "f64", 1, 50), "f32": tfim_leg(35, "f32", 1, 50)}
         extra["tfim_pair"] = tfim_pair_leg(28, 50)
-        extra["batch"] = batch_leg(4096, 20, "f64", 0, 0, 20, 2, 64)
+        extra["batch"] = {"f64": batch_leg(4096, 20, "f64", 0, 0, 20, 2, 64),
+                          "f32": batch_leg(4096, 20, "f32", 0, 0, 20, 2, 64, plain=False)}
         extra["served"] = serve_leg(128, 30, 4, 160.0, 3)
 
     line = {

@@ -409,7 +409,7 @@ typedef struct {
     uint32_t tile_cap[32];    /* tile cap used for registers of each qubit count */
     uint32_t dense_min[32];   /* dense-phase threshold used for each qubit count (0: none) */
     uint32_t tune_caps[3];    /* the tile caps the autotuner tried: 10, 11, 12 (FP64) or 11, 12, 13 (FP32) */
-    uint32_t reserved;
+    uint32_t persist_blocks;  /* blocks of the persistent launch (qvnt_batch_run mode 2); 0: not available */
     double tune_ms[3][3];     /* autotuner: ms per run of the whole batch with one tile cap
                                  (tune_caps) and one dense threshold (0, 3, 4) for every size
                                  (0: not timed) */
@@ -431,10 +431,16 @@ int qvnt_batch_reset(qvnt_batch* b);
  * the registers reset to their initial states. */
 int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qubits, uint32_t dense_min,
                        qvnt_batch_info* info);
-/* Queues one application of every compiled circuit (graph != 0: as one
- * hipGraph, captured at the first such call after a compile). */
+/* Queues one application of every compiled circuit.  graph = 0: the rounds
+ * as separate launches (one per round and tile size); 1: the same captured
+ * once (at the first such call after a compile) into a hipGraph and
+ * replayed; 2: one persistent cooperative launch whose resident blocks walk
+ * all rounds with a grid barrier between them (not captured: it is one
+ * dispatch already); fails with QVNT_E_HIP where info->persist_blocks was 0.
+ * All modes give the same amplitudes bit for bit. */
 int qvnt_batch_run(qvnt_batch* b, int graph);
-/* Runs `runs` times between two device events; *ms = milliseconds per run. */
+/* Runs `runs` times between two device events (graph as qvnt_batch_run);
+ * *ms = milliseconds per run. */
 int qvnt_batch_time(qvnt_batch* b, uint32_t runs, int graph, double* ms);
 int qvnt_batch_sync(qvnt_batch* b);
 /* Register k's amplitudes (2^q interleaved complex of the batch precision). */

@@ -14,7 +14,7 @@ class BatchInfo(ctypes.Structure):
                 ("tiles", ctypes.c_uint64), ("single_ops", ctypes.c_uint64), ("bytes", ctypes.c_double),
                 ("phases", ctypes.c_uint64), ("dense_phases", ctypes.c_uint64), ("dense_flops", ctypes.c_double),
                 ("tile_cap", ctypes.c_uint32 * 32), ("dense_min", ctypes.c_uint32 * 32),
-                ("tune_caps", ctypes.c_uint32 * 3), ("reserved", ctypes.c_uint32),
+                ("tune_caps", ctypes.c_uint32 * 3), ("persist_blocks", ctypes.c_uint32),
                 ("tune_ms", (ctypes.c_double * 3) * 3), ("single_ms", ctypes.c_double * 32)]
 
 
@@ -80,18 +80,26 @@ class Batch:
         return {"rounds": info.rounds, "launches": info.launches, "passes": info.passes, "tiles": info.tiles,
                 "single_ops": info.single_ops, "bytes": info.bytes, "phases": info.phases,
                 "dense_phases": info.dense_phases, "dense_flops": info.dense_flops,
+                "persist_blocks": info.persist_blocks,
                 "tile_cap": {n: info.tile_cap[n] for n in sizes}, "dense_min": {n: info.dense_min[n] for n in sizes},
                 "tune_ms": tune, "single_ms": single}
 
     def reset(self):
         check(lib.qvnt_batch_reset(self._h))
 
+    @staticmethod
+    def _mode(graph):
+        return 2 if graph == "persistent" else (1 if graph else 0)
+
     def run(self, graph=False):
-        check(lib.qvnt_batch_run(self._h, 1 if graph else 0))
+        """One application of every compiled circuit: graph False -- the
+        rounds as launches; True -- the same as one replayed hipGraph;
+        "persistent" -- one persistent cooperative launch over all rounds."""
+        check(lib.qvnt_batch_run(self._h, self._mode(graph)))
 
     def time(self, runs, graph=False):
         ms = ctypes.c_double()
-        check(lib.qvnt_batch_time(self._h, runs, 1 if graph else 0, ctypes.byref(ms)))
+        check(lib.qvnt_batch_time(self._h, runs, self._mode(graph), ctypes.byref(ms)))
         return ms.value
 
     def sync(self):

@@ -27,6 +27,7 @@ using qvnt::fusion::FPass;
 using qvnt::fusion::FPhase;
 using qvnt::fusion::FOp;
 using qvnt::fusion::Program;
+using qvnt::fusion::PUnit;
 using qvnt::fusion::Step;
 
 namespace {
@@ -101,10 +102,17 @@ struct Schedule {
     double bytes = 0.0;    // algorithmic HBM bytes of one run: 2 * amp * 2^n per pass
     void* dprog = nullptr;
     void* dlaunch = nullptr;
+    // the persistent form: work units of every round (k_batch_persist)
+    uint32_t tmax = 0;             // largest tile (the block holds 2^(tmax-4) threads)
+    uint32_t persist_blocks = 0;   // resident blocks of the cooperative launch (0: not available)
+    size_t p_items_off = 0, p_units_off = 0, p_rounds_off = 0;   // bytes into dpersist
+    void* dpersist = nullptr;
     void release() {
         if (dprog) (void)hipFree(dprog);
         if (dlaunch) (void)hipFree(dlaunch);
-        dprog = dlaunch = nullptr;
+        if (dpersist) (void)hipFree(dpersist);
+        dprog = dlaunch = dpersist = nullptr;
+        persist_blocks = 0;
         launches.clear();
     }
 };
@@ -217,7 +225,15 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
         opoff[k] = base[k] + o;
         cfoff[k] = base[k] + c;
     }
-    // rounds: pass r of every register, one launch per tile size
+    // rounds: pass r of every register, one launch per tile size; and the
+    // same work as units of the persistent kernel (up to 2^(tmax - t)
+    // tiles of one register pass per block)
+    S.tmax = 0;
+    for (uint32_t k : sel)
+        for (const FPass& fp : plans[k].prog.passes) S.tmax = std::max(S.tmax, fp.t);
+    std::vector<BItem> p_items;
+    std::vector<PUnit> p_units;
+    std::vector<uint32_t> p_rounds(1, 0);
     std::vector<unsigned char> lhost;
     for (uint32_t r = 0; r < S.rounds; ++r) {
         std::map<uint32_t, std::vector<uint32_t>> by_t;
@@ -241,6 +257,10 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
                 if (map.size() + tiles >= (1ull << 31)) return fail(QVNT_E_ARG, "too many tiles in one round");
                 for (uint64_t j = 0; j < tiles; ++j) map.push_back((uint32_t)items.size());
                 items.push_back(it);
+                const uint64_t per_unit = 1ull << (S.tmax - fp.t);
+                for (uint64_t j = 0; j < tiles; j += per_unit)
+                    p_units.push_back(PUnit{(uint32_t)p_items.size(), (uint32_t)j, (uint32_t)std::min(tiles, j + per_unit), 0u});
+                p_items.push_back(it);
                 S.passes += 1;
                 S.tiles += tiles;
                 S.phases += fp.phase_end - fp.phase_begin;
@@ -259,14 +279,31 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
             std::memcpy(lhost.data() + L.map_off, map.data(), map.size() * sizeof(uint32_t));
             S.launches.push_back(L);
         }
+        p_rounds.push_back((uint32_t)p_units.size());
     }
-    if (hipMalloc(&S.dprog, host.size()) != hipSuccess || hipMalloc(&S.dlaunch, lhost.size() + 256) != hipSuccess) {
+    uint32_t widest = 0;
+    for (uint32_t r = 0; r < S.rounds; ++r) widest = std::max(widest, p_rounds[r + 1] - p_rounds[r]);
+    uint32_t resident = 0;
+    if (S.tmax >= (uint32_t)qvnt::fusion::kMinFused && qvnt::batch_persist_grid(b->prec, S.tmax, &resident) == QVNT_OK)
+        S.persist_blocks = std::min(resident, widest);
+    else
+        qvnt::clear_error();   // the persistent form is optional: the rounds run without it
+    S.p_items_off = 0;
+    S.p_units_off = a256(p_items.size() * sizeof(BItem));
+    S.p_rounds_off = a256(S.p_units_off + p_units.size() * sizeof(PUnit));
+    std::vector<unsigned char> phost(a256(S.p_rounds_off + p_rounds.size() * sizeof(uint32_t)));
+    if (!p_items.empty()) std::memcpy(phost.data() + S.p_items_off, p_items.data(), p_items.size() * sizeof(BItem));
+    if (!p_units.empty()) std::memcpy(phost.data() + S.p_units_off, p_units.data(), p_units.size() * sizeof(PUnit));
+    std::memcpy(phost.data() + S.p_rounds_off, p_rounds.data(), p_rounds.size() * sizeof(uint32_t));
+    if (hipMalloc(&S.dprog, host.size()) != hipSuccess || hipMalloc(&S.dlaunch, lhost.size() + 256) != hipSuccess ||
+        hipMalloc(&S.dpersist, phost.size()) != hipSuccess) {
         (void)hipGetLastError();
         S.release();
         return fail(QVNT_E_OOM, "device memory for the batched programs");
     }
     QVNT_HIP(hipMemcpyAsync(S.dprog, host.data(), host.size(), hipMemcpyHostToDevice, b->stream));
     if (!lhost.empty()) QVNT_HIP(hipMemcpyAsync(S.dlaunch, lhost.data(), lhost.size(), hipMemcpyHostToDevice, b->stream));
+    QVNT_HIP(hipMemcpyAsync(S.dpersist, phost.data(), phost.size(), hipMemcpyHostToDevice, b->stream));
     QVNT_HIP(hipStreamSynchronize(b->stream));  // the host staging buffers die here
     return QVNT_OK;
 }
@@ -280,6 +317,16 @@ int enqueue(qvnt_batch* b, const Schedule& S) {
             return rc;
     }
     return QVNT_OK;
+}
+
+// The whole schedule as one cooperative launch of the persistent kernel.
+int enqueue_persist(qvnt_batch* b, const Schedule& S) {
+    if (!S.persist_blocks) return fail(QVNT_E_HIP, "the persistent batched form is not available on this device");
+    const unsigned char* P = static_cast<const unsigned char*>(S.dpersist);
+    return qvnt::launch_batch_persist(b->stream, b->prec, b->arena, S.dprog, reinterpret_cast<const BItem*>(P + S.p_items_off),
+                                      reinterpret_cast<const PUnit*>(P + S.p_units_off),
+                                      reinterpret_cast<const uint32_t*>(P + S.p_rounds_off), S.rounds, S.tmax,
+                                      S.persist_blocks);
 }
 
 int check_batch(const qvnt_batch* b) {
@@ -495,6 +542,7 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
         info->phases = b->sched.phases;
         info->dense_phases = b->sched.dense_phases;
         info->dense_flops = 128.0 * b->sched.dense_amps;
+        info->persist_blocks = b->sched.persist_blocks;
         for (unsigned n = 0; n <= kMaxQubits && n < 32; ++n) {
             info->tile_cap[n] = caps[n];
             info->dense_min[n] = dense[n];
@@ -511,6 +559,10 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
 int qvnt_batch_run(qvnt_batch* b, int use_graph) {
     if (int rc = check_batch(b)) return rc;
     if (b->sched.launches.empty()) return QVNT_OK;
+    // one persistent launch; never captured: a cooperative launch is
+    // already one dispatch, and stream capture of one is not part of HIP's
+    // documented capture support
+    if (use_graph == 2) return enqueue_persist(b, b->sched);
     if (!use_graph) return enqueue(b, b->sched);
     if (!b->exec) {
         QVNT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

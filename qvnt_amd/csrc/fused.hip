@@ -11,6 +11,8 @@
 // contraction is on in this file; 2x2 products are merged on the host).
 #pragma clang fp contract(fast)
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <cstring>
 #include <vector>
 
@@ -397,6 +399,16 @@ __device__ __forceinline__ uint64_t tile_base(const FPass& pass, uint64_t tile_i
     return tile;
 }
 
+// This thread's swizzled LDS offset in the direct layout, made opaque at
+// its use: the 16 offsets derived from it are used at most twice per tile,
+// and keeping them live across the phases (where the compiler would hoist
+// them) costs 16 VGPRs for the whole kernel.
+__device__ __forceinline__ uint32_t direct_base(uint32_t tid) {
+    uint32_t s = dswz(tid);
+    asm volatile("" : "+v"(s));
+    return s;
+}
+
 // The phases of one pass on one tile.  On entry and on exit v holds the
 // thread's amplitudes in the direct layout (register j = local tid + j *
 // nth); `lds` is the tile's swizzled LDS copy (local element e at dswz(e)).
@@ -405,7 +417,6 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
                                             const FOp* __restrict__ ops, const double* __restrict__ coef, uint64_t tile,
                                             uint32_t tid) {
     const uint32_t nth = pass.nthreads;
-    const uint32_t sdir = dswz(tid);
     const uint32_t p0 = pass.phase_begin, p1 = pass.phase_end;
     for (uint32_t p = p0; p < p1; ++p) {
         const FPhase& ph = phases[p];
@@ -414,14 +425,14 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
             // take the result back out for the final store
             if (p == p0) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
+                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
                 __syncthreads();
             }
             dense_phase<R>(lds, ph, coef + ph.ucoef, tid);
             __syncthreads();
             if (p + 1 == p1) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
+                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
             }
             continue;
         }
@@ -438,7 +449,7 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
         if (!direct_in) {
             if (p == p0) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
+                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) lds[sdir ^ dswz(j * nth)] = v[j];
                 __syncthreads();
             }
 #pragma unroll
@@ -451,7 +462,7 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
             __syncthreads();
             if (p + 1 == p1) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
+                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
             }
         }
     }
@@ -478,33 +489,44 @@ __device__ __forceinline__ void store_direct(cplx<R>* __restrict__ psi, const FP
 
 // One tile of one pass: read 2^t amplitudes, run the pass's phases, write
 // them back; the pass and program live in scalar-loadable memory.
-template <class R>
+// tid: the thread's index within the tile (0 .. 2^(t-4) - 1).  ACTIVE
+// false: the threads run the phases (and their barriers) on their own LDS
+// region but neither read nor write HBM -- a block of the persistent
+// batched kernel whose tile group has no tile (see k_batch_persist).
+template <class R, bool GATED = false>
 __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPass& pass, const FPhase* __restrict__ phases,
                                            const FOp* __restrict__ ops, const double* __restrict__ coef,
-                                           uint64_t tile_index, cplx<R>* lds) {
-    const uint32_t tid = threadIdx.x;
+                                           uint64_t tile_index, cplx<R>* lds, uint32_t tid, bool active = true) {
     const uint64_t tile = tile_base(pass, tile_index);
     const uint64_t base = tile | thread_offset(pass, tid);
     cplx<R> v[kRegs];
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
-        const cplx<R>* p = psi + base + pass.hoff[j];
-        v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+        v[j] = cplx<R>{(R)0, (R)0};
+        if (!GATED || active) {
+            const cplx<R>* p = psi + base + pass.hoff[j];
+            v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+        }
     }
     tile_phases<R>(v, lds, pass, phases, ops, coef, tile, tid);
-    store_direct(psi, pass, base, v);
+    if (!GATED || active) store_direct(psi, pass, base, v);
 }
 
 // At most two waves per SIMD: the 64 KiB LDS tile allows no more, and the
 // whole VGPR file then holds the 16 amplitudes and the op operands
-// without spills.
+// without spills.  (QVNT_FUSED_WAVES, a build-time knob, asks for that many
+// waves per SIMD instead: the VGPR budget shrinks to 512 / waves.)
+#ifdef QVNT_FUSED_WAVES
+#define QVNT_FUSED_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(QVNT_FUSED_WAVES, QVNT_FUSED_WAVES)))
+#else
 #define QVNT_FUSED_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2)))
+#endif
 
 template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
                                                const FOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ unsigned char smem[];
-    fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem));
+    fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
 }
 
 // The batched form: the tiles of one pass of many registers in one launch
@@ -519,7 +541,59 @@ __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const uns
     const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
     fused_tile<R>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
                   reinterpret_cast<const FOp*>(prog + it.op_off), reinterpret_cast<const double*>(prog + it.coef_off),
-                  blockIdx.x - it.block0, reinterpret_cast<cplx<R>*>(smem));
+                  blockIdx.x - it.block0, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
+}
+
+// The batched form as one persistent launch (cooperative: every block is
+// resident).  The blocks walk the rounds in order and meet at a grid
+// barrier between rounds, which also makes a round's HBM writes visible to
+// the next (cooperative groups' grid sync: release and acquire fences at
+// device scope).  Block b runs units b, b + gridDim.x, ... of each round.
+// A block of 2^(tmax-4) threads runs one unit: up to 2^(tmax-t) tiles of
+// one register pass of tile size t, one tile per group of 2^(t-4) threads
+// with its own 2^t-amplitude LDS region; all groups run the same program,
+// so they pass the same barriers, and a group without a tile runs the
+// phases without HBM traffic.  Groups of whole waves (t >= 10) take their
+// tile index wave-uniform, as in k_batch, so the program's tile-dependent
+// branches and table loads stay scalar.
+//
+// The launch bounds are k_fused's: they leave the register allocation at
+// what the kernel uses, which is what the runtime's occupancy query (and so
+// the cooperative grid size) counts.
+template <class R>
+__global__ QVNT_FUSED_BOUNDS void k_batch_persist(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
+                                                  const BItem* __restrict__ items, const PUnit* __restrict__ units,
+                                                  const uint32_t* __restrict__ round_begin, uint32_t rounds) {
+    extern __shared__ unsigned char smem[];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t u_end = round_begin[r + 1];
+        for (uint32_t u = round_begin[r] + blockIdx.x; u < u_end; u += gridDim.x) {
+            const PUnit un = units[u];
+            const BItem it = items[un.item];
+            const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
+            const FPhase* ph = reinterpret_cast<const FPhase*>(prog + it.phase_off);
+            const FOp* ops = reinterpret_cast<const FOp*>(prog + it.op_off);
+            const double* coef = reinterpret_cast<const double*>(prog + it.coef_off);
+            const uint32_t tbits = pass.t - kRegBits;                 // threads per tile: 2^tbits
+            const uint32_t tid = threadIdx.x & ((1u << tbits) - 1u);
+            if (tbits >= 6) {
+                const uint32_t group = __builtin_amdgcn_readfirstlane(threadIdx.x >> tbits);
+                const uint32_t tile = un.tile0 + group;
+                fused_tile<R, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                                    reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+            } else {
+                const uint32_t group = threadIdx.x >> tbits;
+                const uint32_t tile = un.tile0 + group;
+                fused_tile<R, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                                    reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+            }
+            // the next unit's first LDS writes may hit elements other
+            // threads still read in this unit's last phase
+            __syncthreads();
+        }
+        grid.sync();
+    }
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -539,6 +613,38 @@ Layout layout_of(const Program& prog) {
 }  // namespace
 
 size_t program_bytes(const Program& prog) { return layout_of(prog).total; }
+
+int batch_persist_grid(qvnt_prec prec, uint32_t tmax, uint32_t* blocks) {
+    const size_t amp = prec == QVNT_F64 ? 16 : 8;
+    const int threads = 1 << (tmax - kRegBits);
+    const size_t lds = ((size_t)1 << tmax) * amp;
+    int per_cu = 0, dev = 0, cus = 0, coop = 0;
+    QVNT_HIP(hipGetDevice(&dev));
+    QVNT_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) return fail(QVNT_E_HIP, "the device does not support cooperative launches");
+    QVNT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (prec == QVNT_F64)
+        QVNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<double>, threads, lds));
+    else
+        QVNT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<float>, threads, lds));
+    if (per_cu <= 0) return fail(QVNT_E_HIP, "the persistent batched kernel does not fit a compute unit");
+    *blocks = (uint32_t)per_cu * (uint32_t)cus;
+    return QVNT_OK;
+}
+
+int launch_batch_persist(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const BItem* items,
+                         const PUnit* units, const uint32_t* round_begin, uint32_t rounds, uint32_t tmax,
+                         uint32_t blocks) {
+    const size_t amp = prec == QVNT_F64 ? 16 : 8;
+    const dim3 grid(blocks), block(1u << (tmax - kRegBits));
+    const size_t lds = ((size_t)1 << tmax) * amp;
+    const unsigned char* p = static_cast<const unsigned char*>(prog);
+    void* args[] = {&arena, &p, &items, &units, &round_begin, &rounds};
+    const void* fn = prec == QVNT_F64 ? reinterpret_cast<const void*>(k_batch_persist<double>)
+                                      : reinterpret_cast<const void*>(k_batch_persist<float>);
+    QVNT_HIP(hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream));
+    return QVNT_OK;
+}
 
 int launch_batch(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const BItem* items, const uint32_t* map,
                  uint32_t t, uint32_t blocks) {

@@ -135,6 +135,14 @@ struct BItem {              // 32 bytes
 // 16 lanes that differ in bits 4..7 hit 16 different 16-byte bank groups.
 inline uint32_t lds_swizzle(uint32_t e) { return e ^ ((e >> 4) & 15u); }
 
+// A work unit of the persistent batched kernel: tiles [tile0, tile_end) of
+// one register pass (items[item]), at most 2^(tmax - t) of them.
+struct PUnit {              // 16 bytes
+    uint32_t item;
+    uint32_t tile0, tile_end;
+    uint32_t pad;
+};
+
 struct Program {
     std::vector<FPass> passes;
     std::vector<FPhase> phases;

@@ -103,5 +103,13 @@ namespace fusion { struct BItem; }
 // one batched launch: `blocks` tiles of 2^t amplitudes (block -> item by map)
 int launch_batch(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const fusion::BItem* items,
                  const uint32_t* map, uint32_t t, uint32_t blocks);
+namespace fusion { struct PUnit; }
+// the persistent batched kernel: blocks resident at once for tiles up to
+// 2^tmax, and its cooperative launch over `rounds` rounds of work units
+// (units [round_begin[r], round_begin[r+1]) form round r; device pointers)
+int batch_persist_grid(qvnt_prec prec, uint32_t tmax, uint32_t* blocks);
+int launch_batch_persist(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const fusion::BItem* items,
+                         const fusion::PUnit* units, const uint32_t* round_begin, uint32_t rounds, uint32_t tmax,
+                         uint32_t blocks);
 
 }  // namespace qvnt

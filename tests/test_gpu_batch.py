@@ -101,6 +101,32 @@ def test_graph_replays_match_eager():
         assert np.array_equal(twice, r.get_state()[: 1 << n])
 
 
+@pytest.mark.parametrize("prec,cap,dense", [("f64", 12, 0), ("f64", 12, 2), ("f64", 9, 0), ("f32", 13, 3),
+                                             ("f32", 10, 0)])
+def test_persistent_launch_equals_rounds(prec, cap, dense):
+    """The persistent cooperative launch (all rounds in one dispatch, a grid
+    barrier between rounds) gives the round launches' amplitudes bit for
+    bit.  Registers of 6..16 qubits under one tile cap mix tile sizes in a
+    round, so a block runs several tiles of a small register at once; two
+    runs in a row, twice after a reset."""
+    regs, states, ops = workload(48, 31 + cap + dense, n_lo=6, n_hi=16 if dense else 14)
+    with Batch([n for n, _ in regs], states, prec=prec) as b:
+        info = b.compile(ops, max_qubits=cap, dense_min=dense)
+        assert info["persist_blocks"] > 0
+        if dense:
+            assert info["dense_phases"] > 0
+        b.run()
+        b.run()
+        want = [b.get_state(k) for k in range(len(regs))]
+        for rep in range(2):
+            b.reset()
+            b.run(graph="persistent")
+            b.run(graph="persistent")
+            b.sync()
+            for k, (n, _) in enumerate(regs):
+                assert np.array_equal(b.get_state(k), want[k]), "run %d, register %d (%d qubits)" % (rep, k, n)
+
+
 @pytest.mark.parametrize("dense,cap", [(0, 12), (2, 12), (0, 13), (3, 13)])
 def test_fp32_batch_error_bound(dense, cap):
     regs, states, ops = workload(24, 9 + dense + cap, n_lo=8, n_hi=16)
@@ -140,3 +166,27 @@ def test_batch_rejects_bad_input():
             b.compile([op.x(1), op.x(1 << 7)])     # qubit 7 beyond a 7-qubit register
         with pytest.raises(TypeError):
             b.compile([op.x(1)])
+
+
+def test_batch_at_config_sizes():
+    """512 registers of 10-20 qubits (the batched workload's sizes, 20-120
+    gates each), autotuned and run as a hipGraph: every 16th register equals
+    the single-register fused form with the settings the autotuner chose,
+    bit for bit, and every register keeps its norm."""
+    regs, states, ops = workload(512, 4242, n_lo=10, n_hi=20, g_lo=20, g_hi=120)
+    with Batch([n for n, _ in regs], states) as b:
+        info = b.compile(ops)
+        b.run(graph=True)
+        b.sync()
+        for k in range(0, len(regs), 16):
+            n = regs[k][0]
+            with QReg.with_state(n, states[k]) as r:
+                r.set_dense(info["dense_min"][n])
+                r.apply_fused(ops[k], max_qubits=info["tile_cap"][n])
+                want = r.get_state()[: 1 << n]
+            got = b.get_state(k)
+            if n < 20:
+                assert np.array_equal(got, want), "register %d (%d qubits)" % (k, n)
+            else:   # from 2^20 amplitudes the single-register form runs one-gate passes as plain launches
+                assert np.max(np.abs(got - want)) <= 1e-12, "register %d (%d qubits)" % (k, n)
+            assert abs(np.vdot(got, got).real - 1.0) < 1e-12

@@ -1,6 +1,7 @@
 """Register allocation of the fused kernels, from the compiler's own
 resource report: the 16 amplitudes each thread holds must stay in VGPRs
-(no scratch memory), for both precisions and both launch forms."""
+(no scratch memory), for both precisions and all three launch forms
+(k_fused, k_batch, k_batch_persist)."""
 import os
 import re
 import shutil
@@ -31,5 +32,34 @@ def test_fused_kernels_do_not_spill(tmp_path):
         if m and name:
             kernels[name] = int(m.group(1))
     fused = {k: v for k, v in kernels.items() if "k_fused" in k or "k_batch" in k}
-    assert len(fused) == 4, kernels
+    assert len(fused) == 6, kernels
     assert all(v == 0 for v in fused.values()), fused
+
+
+@pytest.mark.skipif(not os.path.exists(build.HIPCC), reason="needs hipcc")
+def test_cooperative_kernel_allocates_what_its_metadata_says(tmp_path):
+    """The persistent batched kernel is launched cooperatively with as many
+    blocks as the runtime's occupancy query allows, and that query counts
+    the code object's metadata (.vgpr_count, .agpr_count).  The hardware
+    allocates what the kernel descriptor asks for (next_free_vgpr).  If the
+    compiler pads the allocation beyond the metadata's count (it does so to
+    honour a minimum of waves per SIMD), fewer blocks fit than were
+    launched and the grid barrier never completes.  So the two must agree."""
+    cmd = build.compile_command("fused.hip", str(tmp_path / "fused.o")) + ["--save-temps"]
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=600, cwd=str(tmp_path))
+    assert res.returncode == 0, res.stderr
+    asm = [p for p in os.listdir(tmp_path) if p.endswith(".s") and "gfx950" in p]
+    assert len(asm) == 1, os.listdir(tmp_path)
+    text = open(os.path.join(tmp_path, asm[0])).read()
+    descriptors = dict(re.findall(r"\.amdhsa_kernel (\S+)\n(?:.*\n)*?\s*\.amdhsa_next_free_vgpr (\d+)", text))
+    checked = 0
+    for block in text.split("  - .agpr_count:")[1:]:
+        name = re.search(r"\.name:\s+(\S+)", block).group(1)
+        if "k_batch_persist" not in name:
+            continue
+        agpr = int(block.split("\n", 1)[0])
+        vgpr = int(re.search(r"\.vgpr_count:\s+(\d+)", block).group(1))
+        assert agpr == 0, (name, agpr)
+        assert int(descriptors[name]) == vgpr, (name, descriptors[name], vgpr)
+        checked += 1
+    assert checked == 2
