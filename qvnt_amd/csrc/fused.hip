@@ -409,6 +409,23 @@ __device__ __forceinline__ uint32_t direct_base(uint32_t tid) {
     return s;
 }
 
+// The tile's final read-out from LDS into the direct layout (register j =
+// local tid + j * nth), through the pass's qubit permutation if it has one.
+template <class R>
+__device__ __forceinline__ void read_out(cplx<R>* v, const cplx<R>* lds, const FPass& pass, uint32_t tid) {
+    if (pass.permuted) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k + kRegBits < pass.t; ++k)
+            if (tid >> k & 1) s ^= pass.pimg[k];
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) v[j] = lds[s ^ pass.pout[j]];
+    } else {
+        const uint32_t nth = pass.nthreads;
+#pragma unroll
+        for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
+    }
+}
+
 // The phases of one pass on one tile.  On entry and on exit v holds the
 // thread's amplitudes in the direct layout (register j = local tid + j *
 // nth); `lds` is the tile's swizzled LDS copy (local element e at dswz(e)).
@@ -430,10 +447,7 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
             }
             dense_phase<R>(lds, ph, coef + ph.ucoef, tid);
             __syncthreads();
-            if (p + 1 == p1) {
-#pragma unroll
-                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
-            }
+            if (p + 1 == p1) read_out<R>(v, lds, pass, tid);
             continue;
         }
         // local base of this thread in the phase: zeros at the register bits
@@ -460,10 +474,7 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
 #pragma unroll
             for (int i = 0; i < kRegs; ++i) lds[stb ^ ph.sroff[i]] = v[i];
             __syncthreads();
-            if (p + 1 == p1) {
-#pragma unroll
-                for (int j = 0, sdir = direct_base(tid); j < kRegs; ++j) v[j] = lds[sdir ^ dswz(j * nth)];
-            }
+            if (p + 1 == p1) read_out<R>(v, lds, pass, tid);
         }
     }
 }

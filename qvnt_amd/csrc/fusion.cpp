@@ -32,6 +32,20 @@ struct Prim {
     }
 };
 
+// An uncontrolled swap of qubits q0 and q1, taken out of the queue: the
+// compiler relabels the qubits of every later primitive instead of moving
+// amplitudes (see relabel_swaps).
+constexpr uint8_t kRelabel = 0xF0;
+
+// The primitive with every qubit q moved to to[q].
+void remap(Prim& p, const int* to) {
+    p.q0 = to[p.q0];
+    if (p.q1 >= 0) p.q1 = to[p.q1];
+    uint64_t c = 0;
+    for (uint64_t m = p.ctrl; m; m &= m - 1) c |= 1ull << to[ctz(m)];
+    p.ctrl = c;
+}
+
 constexpr double kR = 0.70710678118654752440;
 
 Prim m2(int q, cd a, cd b, cd c, cd d, uint64_t ctrl) {
@@ -151,6 +165,21 @@ bool lower(const qvnt_gate& g, uint64_t lmask, std::vector<Prim>& out) {
     }
     // two-qubit matrices, index bit 0 = lower qubit
     const std::vector<int> b = bitlist(g.kind == QVNT_H2 || g.kind == QVNT_U2 ? (g.a | g.b) : g.a);
+    if ((g.kind == QVNT_SWAP || g.kind == QVNT_ISWAP) && ctrl == 0) {
+        // iswap = swap * diag(1, w, w, 1) (w = i, or -i daggered): the
+        // diagonal is symmetric in the two qubits, so it stays a phase
+        // table on them, and the swap becomes a relabeling
+        if (g.kind == QVNT_ISWAP) {
+            const cd w = g.dagger ? -I : I;
+            push_diag2(out, b[0], b[1], 1.0, w, w, 1.0, 0);
+        }
+        Prim r;
+        r.type = kRelabel;
+        r.q0 = b[0];
+        r.q1 = b[1];
+        out.push_back(r);
+        return true;
+    }
     cd u[16];
     for (int k = 0; k < 16; ++k) u[k] = 0.0;
     auto set = [&](int r, int col, cd v) { u[4 * r + col] = v; };
@@ -244,6 +273,11 @@ std::vector<Prim> merge_single(const std::vector<Prim>& in) {
 struct PassPlan {
     uint64_t T = 0;
     std::vector<Prim> prims;
+    // the pass ends by moving the amplitude of index bit x (x in T) to bit
+    // sigma[x]: a permutation of T's qubits, free in the kernel (the tile's
+    // last LDS read-out is permuted)
+    bool permuted = false;
+    int8_t sigma[64];
 };
 
 // The primitives left behind so far in a scan of the queue.  A later
@@ -282,26 +316,85 @@ unsigned row_bits(unsigned amp_bytes) {
 // touches no qubit of T (in this pass it would only scale whole tiles; in
 // a later one its qubits may be registers), provided the pass keeps some
 // matrix primitive, so every pass makes progress.
-std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns, unsigned tq, unsigned rows) {
+//
+// want (relabeled queues, see relabel_swaps): the primitives address the
+// qubit layout in which qubit f's amplitudes belong at index bit want[f]
+// once the queue is done.  Each pass then also permutes the qubits of its
+// set T (PassPlan::sigma), moving every amplitude bit whose home is in T
+// there; T's free slots are filled with qubits that let it move more; the
+// primitives left are relabeled to follow; and passes without primitives
+// finish what is still out of place.
+std::vector<PassPlan> schedule_passes(std::vector<Prim> work, unsigned ns, unsigned tq, unsigned rows,
+                                      const std::vector<int>* want) {
     std::vector<PassPlan> passes;
     const unsigned rb = ns < rows ? ns : rows;
     const uint64_t forced = (1ull << rb) - 1u;
-    // Once the left-behind matrix primitives target every qubit the queue
-    // touches, every later primitive is blocked: a scan stops there.
-    uint64_t support = 0;
-    for (const Prim& p : all) support |= p.bits();
-    // the queue holds indices into `all`, in circuit order
-    std::vector<uint32_t> rest(all.size()), leave, take;
-    for (size_t k = 0; k < all.size(); ++k) rest[k] = (uint32_t)k;
+    // the queue holds indices into `work`, in circuit order
+    std::vector<uint32_t> rest(work.size()), leave, take;
+    for (size_t k = 0; k < work.size(); ++k) rest[k] = (uint32_t)k;
     // does a matrix primitive after queue position i target one of `bits`?
     auto later_meets = [&](size_t i, uint64_t bits) {
         for (size_t k = i + 1; k < rest.size(); ++k) {
-            const Prim& q = all[rest[k]];
+            const Prim& q = work[rest[k]];
             if (q.type != OP_D && (q.targets() & bits)) return true;
         }
         return false;
     };
+    // layout: at[x] = the qubit of `work`'s original layout whose amplitude
+    // bit now sits at index bit x; home(x) = where that bit belongs
+    std::vector<int> at(ns), where(ns);
+    for (unsigned q = 0; q < ns; ++q) at[q] = where[q] = (int)q;
+    auto home = [&](int x) { return want ? (*want)[(size_t)at[(size_t)x]] : x; };
+    auto out_of_place = [&]() {
+        for (unsigned x = 0; x < ns; ++x)
+            if (home((int)x) != (int)x) return true;
+        return false;
+    };
+    // the bits that belong in T or sit in T, while T has room
+    auto pad_moves = [&](uint64_t& T) {
+        if (!want) return;
+        for (unsigned x = 0; x < ns && popc(T) < tq; ++x)
+            if ((T >> x & 1) && home((int)x) != (int)x) T |= 1ull << home((int)x);
+        for (unsigned x = 0; x < ns && popc(T) < tq; ++x)
+            if (!(T >> x & 1) && (T >> home((int)x) & 1)) T |= 1ull << x;
+        for (unsigned x = 0; x < ns && popc(T) + 2 <= tq; ++x)
+            if (!(T >> x & 1) && home((int)x) != (int)x && !(T >> home((int)x) & 1))
+                T |= (1ull << x) | (1ull << home((int)x));
+    };
+    // the pass's permutation of T: every bit whose home is in T goes home,
+    // the others fill T's remaining bits in order; then the layout and the
+    // primitives still queued follow it
+    auto permute = [&](PassPlan& pp) {
+        for (int x = 0; x < 64; ++x) pp.sigma[x] = (int8_t)x;
+        if (!want) return;
+        uint64_t placed = 0, taken = 0;
+        for (uint64_t m = pp.T; m; m &= m - 1) {
+            const int x = (int)ctz(m), h = home(x);
+            if (pp.T >> h & 1) {
+                pp.sigma[x] = (int8_t)h;
+                placed |= 1ull << x;
+                taken |= 1ull << h;
+            }
+        }
+        uint64_t src = pp.T & ~placed, dst = pp.T & ~taken;
+        for (; src; src &= src - 1, dst &= dst - 1) pp.sigma[ctz(src)] = (int8_t)ctz(dst);
+        int to[64];
+        for (int x = 0; x < 64; ++x) {
+            to[x] = pp.sigma[x];
+            if (to[x] != x) pp.permuted = true;
+        }
+        if (!pp.permuted) return;
+        std::vector<int> moved(ns);
+        for (unsigned x = 0; x < ns; ++x) moved[(size_t)to[x]] = at[x];
+        at.swap(moved);
+        for (unsigned x = 0; x < ns; ++x) where[(size_t)at[x]] = (int)x;
+        for (uint32_t k : rest) remap(work[k], to);
+    };
     while (!rest.empty()) {
+        // Once the left-behind matrix primitives target every qubit the
+        // queue touches, every later primitive is blocked: a scan stops there.
+        uint64_t support = 0;
+        for (uint32_t k : rest) support |= work[k].bits();
         PassPlan pp;
         pp.T = forced;
         Left left;
@@ -310,14 +403,16 @@ std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns,
         take.clear();
         size_t i = 0;
         for (; i < rest.size() && (left.mat_targets & support) != support; ++i) {
-            const Prim& p = all[rest[i]];
+            const Prim& p = work[rest[i]];
             if (left.blocks(p)) { left.add(p); leave.push_back(rest[i]); continue; }
             const uint64_t nt = pp.T | p.targets();
             if (popc(nt) <= tq) { pp.T = nt; take.push_back(rest[i]); matrix |= p.type != OP_D; }
             else { left.add(p); leave.push_back(rest[i]); }
         }
         leave.insert(leave.end(), rest.begin() + (long)i, rest.end());
-        // pad T with the lowest free qubits up to tq (larger, fewer tiles)
+        // pad T up to tq: first with bits a permutation can bring home,
+        // then with the lowest free qubits (larger, fewer tiles)
+        pad_moves(pp.T);
         for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
         if (matrix) {
             // a deferred diagonal must meet a later matrix target, so some
@@ -326,7 +421,7 @@ std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns,
             leave.clear();
             left = Left();
             for (i = 0; i < rest.size() && (left.mat_targets & support) != support; ++i) {
-                const Prim& p = all[rest[i]];
+                const Prim& p = work[rest[i]];
                 if (left.blocks(p) || (p.targets() & ~pp.T) ||
                     (p.type == OP_D && !(p.bits() & pp.T) && later_meets(i, p.bits()))) {
                     left.add(p);
@@ -338,9 +433,30 @@ std::vector<PassPlan> schedule_passes(const std::vector<Prim>& all, unsigned ns,
             leave.insert(leave.end(), rest.begin() + (long)i, rest.end());
         }
         pp.prims.reserve(take.size());
-        for (uint32_t k : take) pp.prims.push_back(all[k]);
-        passes.push_back(std::move(pp));
+        for (uint32_t k : take) pp.prims.push_back(work[k]);
         rest.swap(leave);
+        permute(pp);
+        passes.push_back(std::move(pp));
+    }
+    // what is still out of place: passes that only permute, each taking
+    // whole cycles of the layout (or as much of one as fits).  Each brings
+    // at least one bit home (the first cycle's first two bits fit: the
+    // caller relabels only where tq >= rows + 2 or T holds every qubit), so
+    // at most ns such passes follow.
+    while (want && out_of_place()) {
+        PassPlan pp;
+        pp.T = forced;
+        for (unsigned x0 = 0; x0 < ns && popc(pp.T) < tq; ++x0) {
+            if (home((int)x0) == (int)x0 || (pp.T >> x0 & 1)) continue;
+            int x = (int)x0;
+            do {
+                pp.T |= 1ull << x;
+                x = home(x);
+            } while (x != (int)x0 && popc(pp.T) < tq);
+        }
+        for (unsigned q = 0; q < ns && popc(pp.T) < tq; ++q) pp.T |= 1ull << q;
+        permute(pp);
+        passes.push_back(std::move(pp));
     }
     return passes;
 }
@@ -994,6 +1110,11 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
         plans.push_back(std::move(plan));
         rest.swap(leave);
     }
+    if (plans.empty()) {  // a pass that only permutes: one phase without ops
+        PhasePlan plan;
+        plan.Rl = top_regs;
+        plans.push_back(std::move(plan));
+    }
     relocate_diagonals(plans, fp);
 
     fp.phase_begin = (uint32_t)prog.phases.size();
@@ -1130,6 +1251,24 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     };
     fp.direct_first = is_top(prog.phases[fp.phase_begin]) ? 1u : 0u;
     fp.direct_last = is_top(prog.phases[fp.phase_end - 1]) ? 1u : 0u;
+    if (pp.permuted) {
+        // the qubit permutation as a permuted LDS read-out of the tile: the
+        // amplitude at local index e belongs at lam(e) (lam moves local bit
+        // l, index bit tpos[l], to the local bit of index bit sigma[tpos[l]]),
+        // so the thread that stores local index L reads local index
+        // lam^-1(L), whose swizzled offset is XOR-linear in L's bits
+        int inv[kMaxTile];
+        for (uint32_t l = 0; l < fp.t; ++l) inv[local_of(fp.tpos, fp.t, pp.sigma[fp.tpos[l]])] = (int)l;
+        fp.permuted = 1;
+        fp.direct_last = 0;
+        for (uint32_t k = 0; k + kRegBits < fp.t; ++k) fp.pimg[k] = lds_swizzle(1u << inv[k]);
+        for (int j = 0; j < kRegs; ++j) {
+            uint32_t e = 0;
+            for (int b = 0; b < kRegBits; ++b)
+                if (j >> b & 1) e |= 1u << inv[fp.t - kRegBits + b];
+            fp.pout[j] = lds_swizzle(e);
+        }
+    }
     prog.passes.push_back(fp);
 }
 
@@ -1233,6 +1372,51 @@ unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::ve
     return kAutoTiles[best];
 }
 
+namespace {
+
+// Takes the swap marks out of a segment's primitives: an uncontrolled swap
+// of qubits a and b exchanges their amplitude bits, so instead of moving
+// amplitudes the compiler exchanges the two qubits' positions for every
+// later primitive.  Afterwards qubit q's bit sits at position pos[q];
+// *want (indexed by position) says where each position's bit belongs.
+// Returns false (want untouched) if the segment has no swap.  With
+// `keep`, the marks become swap matrices again instead (tiles too small
+// for the permutation passes).
+bool relabel_swaps(std::vector<Prim>& prims, unsigned ns, bool keep, std::vector<int>* want) {
+    bool any = false;
+    for (const Prim& p : prims) any |= p.type == kRelabel;
+    if (!any) return false;
+    if (keep) {
+        for (Prim& p : prims) {
+            if (p.type != kRelabel) continue;
+            cd u[16] = {};
+            u[0] = 1.0; u[4 * 1 + 2] = 1.0; u[4 * 2 + 1] = 1.0; u[15] = 1.0;
+            const long src = p.src;
+            p = m4(p.q0, p.q1, u, 0);
+            p.src = src;
+        }
+        return false;
+    }
+    int pos[64];
+    for (int q = 0; q < 64; ++q) pos[q] = q;
+    std::vector<Prim> out;
+    out.reserve(prims.size());
+    for (Prim p : prims) {
+        if (p.type == kRelabel) {
+            std::swap(pos[p.q0], pos[p.q1]);
+            continue;
+        }
+        remap(p, pos);
+        out.push_back(p);
+    }
+    prims.swap(out);
+    want->assign(ns, 0);
+    for (unsigned q = 0; q < ns; ++q) (*want)[(size_t)pos[q]] = (int)q;
+    return true;
+}
+
+}  // namespace
+
 void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<Step>& steps, unsigned dense_min,
           bool single_plain, bool parallel, unsigned amp_bytes) {
     const uint64_t lmask = (1ull << ns) - 1u;
@@ -1248,7 +1432,11 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
             for (size_t k = before; k < prims.size(); ++k) prims[k].src = (long)j;
         }
         if (!prims.empty()) {
-            const std::vector<PassPlan> passes = schedule_passes(merge_single(prims), ns, tq, row_bits(amp_bytes));
+            const unsigned rows = row_bits(amp_bytes);
+            std::vector<int> want;
+            const bool relabeled = relabel_swaps(prims, ns, ns > tq && tq < rows + 2, &want);
+            const std::vector<PassPlan> passes =
+                schedule_passes(merge_single(prims), ns, tq, rows, relabeled ? &want : nullptr);
             // primitives per source SingleOp, to find passes that hold one
             // whole SingleOp and nothing else: one plain launch moves at
             // most the same bytes as a pass (fewer with controls) and skips
@@ -1262,7 +1450,9 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
             for (size_t k = 0; k < passes.size(); ++k) {
                 const PassPlan& pp = passes[k];
                 const long src = pp.prims.empty() ? -1 : pp.prims[0].src;
-                bool single = single_plain && src >= 0 && per_src[(size_t)src] == pp.prims.size();
+                // (not in a relabeled segment: its primitives address
+                // moved qubits, the SingleOp its original ones)
+                bool single = single_plain && !relabeled && src >= 0 && per_src[(size_t)src] == pp.prims.size();
                 for (const Prim& p : pp.prims) single = single && p.src == src;
                 if (single) single_src[k] = src;
                 else work += pp.prims.size();

@@ -114,9 +114,15 @@ struct FPass {
     uint32_t nthreads;        // 2^(t - 4)
     uint32_t phase_begin, phase_end;
     uint32_t direct_first;    // 1: the first phase's registers are local bits t-4..t-1
-    uint32_t direct_last;     // 1: same for the last phase
+    uint32_t direct_last;     // 1: same for the last phase (0 when permuted)
     uint32_t tpos[kMaxTile];  // global positions of the local bits, ascending
     uint64_t hoff[kRegs];     // global offset of local bits t-4..t-1 = j
+    // permuted: the pass ends with a permutation of its qubits (a relabeled
+    // swap put back): the thread that stores local index L = tid + j * 2^(t-4)
+    // reads it from LDS offset (XOR over tid's set bits k of pimg[k]) ^ pout[j]
+    uint32_t permuted;
+    uint32_t pimg[kMaxTile - kRegBits];
+    uint32_t pout[kRegs];
 };
 
 // One register's pass in a batched launch: byte offsets into the program

@@ -127,6 +127,31 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
 }
 
 // One pass over the state: every tile in turn (fused.hip: fused_tile).
+// The tile's last read-out from LDS into the direct layout, through the
+// pass's qubit permutation when it has one (the kernel's read_out); a
+// permuted read-out must visit every LDS element exactly once.
+static void read_out(std::vector<C>& V, const std::vector<C>& lds, const FPass& pass) {
+    const uint32_t t = pass.t, nth = pass.nthreads;
+    if (!pass.permuted) {
+        for (uint32_t tid = 0; tid < nth; ++tid)
+            for (int j = 0; j < kRegs; ++j) V[tid * kRegs + j] = lds[swz(tid) ^ swz(j * nth)];
+        return;
+    }
+    CHECK(!pass.direct_last);
+    std::vector<char> seen(1u << t, 0);
+    for (uint32_t tid = 0; tid < nth; ++tid) {
+        uint32_t s = 0;
+        for (uint32_t k = 0; k + kRegBits < t; ++k)
+            if (tid >> k & 1) s ^= pass.pimg[k];
+        for (int j = 0; j < kRegs; ++j) {
+            const uint32_t off = s ^ pass.pout[j];
+            CHECK(off < (1u << t) && !seen[off]);
+            seen[off] = 1;
+            V[tid * kRegs + j] = lds[off];
+        }
+    }
+}
+
 static void run_pass(C* psi, uint64_t len, const Program& P, const FPass& pass) {
     const uint32_t t = pass.t, nth = pass.nthreads;
     CHECK(t >= 4 && t <= (uint32_t)kMaxTile && nth == (1u << (t - kRegBits)));
@@ -185,9 +210,7 @@ static void run_pass(C* psi, uint64_t len, const Program& P, const FPass& pass) 
                         lds.at(swz(tb) ^ swz(roff[m])) = acc;
                     }
                 }
-                if (p + 1 == pass.phase_end)
-                    for (uint32_t tid = 0; tid < nth; ++tid)
-                        for (int j = 0; j < kRegs; ++j) V[tid * kRegs + j] = lds[swz(tid) ^ swz(j * nth)];
+                if (p + 1 == pass.phase_end) read_out(V, lds, pass);
                 continue;
             }
             for (uint32_t tid = 0; tid < nth; ++tid) {
@@ -207,9 +230,7 @@ static void run_pass(C* psi, uint64_t len, const Program& P, const FPass& pass) 
                 else
                     for (int i = 0; i < kRegs; ++i) CHECK((tb | roff[i]) == tid + i * nth);
             }
-            if (!dout && p + 1 == pass.phase_end)
-                for (uint32_t tid = 0; tid < nth; ++tid)
-                    for (int j = 0; j < kRegs; ++j) V[tid * kRegs + j] = lds[swz(tid) ^ swz(j * nth)];
+            if (!dout && p + 1 == pass.phase_end) read_out(V, lds, pass);
         }
         for (uint32_t tid = 0; tid < nth; ++tid)
             for (int j = 0; j < kRegs; ++j) psi[tile | lo_off[tid] | pass.hoff[j]] = V[tid * kRegs + j];
@@ -303,7 +324,9 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
             const FPass &u = p.passes[i], &v = q.passes[i];
             if (u.t != v.t || u.nthreads != v.nthreads || u.phase_begin != v.phase_begin ||
                 u.phase_end != v.phase_end || u.direct_first != v.direct_first || u.direct_last != v.direct_last ||
-                std::memcmp(u.tpos, v.tpos, sizeof u.tpos) != 0 || std::memcmp(u.hoff, v.hoff, sizeof u.hoff) != 0)
+                std::memcmp(u.tpos, v.tpos, sizeof u.tpos) != 0 || std::memcmp(u.hoff, v.hoff, sizeof u.hoff) != 0 ||
+                u.permuted != v.permuted || std::memcmp(u.pimg, v.pimg, sizeof u.pimg) != 0 ||
+                std::memcmp(u.pout, v.pout, sizeof u.pout) != 0)
                 return false;
         }
         if (std::memcmp(p.phases.data(), q.phases.data(), p.phases.size() * sizeof(FPhase)) != 0 ||
@@ -346,7 +369,7 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
     for (auto& z : psi) z = C(nd(g), nd(g));
     std::memcpy(orc_reg_psi(ref), psi.data(), len * sizeof(C));
     orc_reg_apply(ref, gates.data(), gates.size());
-    unsigned passes = 0, dense_phases = 0, split = 0, plain = 0;
+    unsigned passes = 0, dense_phases = 0, split = 0, plain = 0, permuted = 0;
     for (const Step& st : steps) {
         if (st.plain) {
             ++plain;
@@ -362,13 +385,15 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
         for (const FPass& fp : st.prog.passes) {
             run_pass(psi.data(), len, st.prog, fp);
             ++passes;
+            permuted += fp.permuted;
         }
     }
     const C* want = reinterpret_cast<const C*>(orc_reg_psi(ref));
     double err = 0.0;
     for (uint64_t i = 0; i < len; ++i) err = std::max(err, std::abs(psi[i] - want[i]));
-    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes, %u plain steps, %u dense phases, %u split/structured ops, "
-                "max |d| %.2e\n", n, tq, dense, gates.size(), passes, plain, dense_phases, split, err);
+    std::printf("n=%u tq=%u dense=%u: %zu gates, %u passes (%u permuted), %u plain steps, %u dense phases, "
+                "%u split/structured ops, max |d| %.2e\n", n, tq, dense, gates.size(), passes, permuted, plain,
+                dense_phases, split, err);
     if (!(err <= 1e-12)) ++failures;
     orc_reg_free(ref);
 }
