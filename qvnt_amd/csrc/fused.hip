@@ -161,6 +161,26 @@ __device__ __forceinline__ void apply_p4(cplx<R>* v, const double* __restrict__ 
     }
 }
 
+// P4 with real diagonal and imaginary pairing coefficients (fusion.hpp:
+// OP_P4R): out_r = c_r x_r + i s_r x_(r^3)
+template <int J0, int J1, class R>
+__device__ __forceinline__ void apply_p4r(cplx<R>* v, const double* __restrict__ c, uint32_t pc) {
+#pragma unroll
+    for (int i = 0; i < kRegs; ++i) {
+        if (i & ((1 << J0) | (1 << J1))) continue;
+        if (!((pc >> i) & 1u)) continue;
+        const int idx[4] = {i, i | (1 << J0), i | (1 << J1), i | (1 << J0) | (1 << J1)};
+        cplx<R> x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const cplx<R> k = ld_coef<R>(c, r);
+            v[idx[r]] = cplx<R>{k.re * x[r].re - k.im * x[r ^ 3].im, k.re * x[r].im + k.im * x[r ^ 3].re};
+        }
+    }
+}
+
 // diagonal: one phase per register (host tables, controls folded in)
 template <class R>
 __device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c) {
@@ -271,6 +291,15 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
             case 6: apply_p4<1, 2>(v, c, op.pc); break;
             case 7: apply_p4<1, 3>(v, c, op.pc); break;
             default: apply_p4<2, 3>(v, c, op.pc); break;
+            }
+        } else if (op.type == OP_P4R) {
+            switch (op.sel) {
+            case 1: apply_p4r<0, 1>(v, c, op.pc); break;
+            case 2: apply_p4r<0, 2>(v, c, op.pc); break;
+            case 3: apply_p4r<0, 3>(v, c, op.pc); break;
+            case 6: apply_p4r<1, 2>(v, c, op.pc); break;
+            case 7: apply_p4r<1, 3>(v, c, op.pc); break;
+            default: apply_p4r<2, 3>(v, c, op.pc); break;
             }
         } else if (op.type == OP_A2) {
             if (op.pc == 0xFFFFu) {

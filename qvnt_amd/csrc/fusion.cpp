@@ -555,6 +555,8 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
                 for (int c = 0; c < 4; ++c) {
                     cd x = 0.0;
                     if (op.type == OP_M4) x = cf(4 * r + c);
+                    else if (op.type == OP_P4R && c == r) x = cf(r).real();
+                    else if (op.type == OP_P4R && c == (r ^ 3)) x = cd(0.0, cf(r).imag());
                     else if (c == r) x = cf(2 * r);
                     else if (c == (r ^ 3)) x = cf(2 * r + 1);
                     v[4 * r + c] = x;
@@ -573,7 +575,7 @@ void dense_of(const Program& prog, uint32_t op_begin, uint32_t op_end, cd* U) {
 
 uint32_t reg_targets(const FOp& op) {
     if (op.type == OP_D) return 0;
-    if (op.type == OP_M4 || op.type == OP_P4) return (1u << (op.sel / 4)) | (1u << (op.sel % 4));
+    if (op.type == OP_M4 || op.type == OP_P4 || op.type == OP_P4R) return (1u << (op.sel / 4)) | (1u << (op.sel % 4));
     return 1u << op.sel;
 }
 
@@ -817,6 +819,7 @@ uint32_t op_cost(const FOp& op) {
     case OP_M2: per = 8; break;
     case OP_M4: per = 16; break;
     case OP_P4: per = 8; break;
+    case OP_P4R: per = 4; break;
     case OP_A2: per = 4; break;
     case OP_X2: per = 1; break;
     case OP_G: per = 0; break;
@@ -887,6 +890,7 @@ uint32_t coef_complex(const FOp& op) {
     case OP_M2: return 4;
     case OP_M4: return 16;
     case OP_P4: return 8;
+    case OP_P4R: return 4;
     case OP_A2: return 2;
     case OP_X2: return 0;
     case OP_S: return 2 * kRegs + kRegBits;
@@ -1214,7 +1218,15 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
                 for (int r = 0; r < 4; ++r)
                     for (int c = 0; c < 4; ++c)
                         if (c != r && c != (r ^ 3) && u[4 * r + c] != 0.0) paired = false;
-                if (paired) {
+                bool real_imag = paired;  // real diagonal, imaginary pairing (rxx, ryy)
+                for (int r = 0; r < 4 && real_imag; ++r)
+                    real_imag = u[4 * r + r].imag() == 0.0 && u[4 * r + (r ^ 3)].real() == 0.0;
+                if (real_imag) {
+                    op.type = OP_P4R;
+                    cd pr[4];
+                    for (int r = 0; r < 4; ++r) pr[r] = cd(u[4 * r + r].real(), u[4 * r + (r ^ 3)].imag());
+                    op.coef = add_coef(prog, pr, 4);
+                } else if (paired) {
                     op.type = OP_P4;
                     cd pr[8];
                     for (int r = 0; r < 4; ++r) {

@@ -69,15 +69,18 @@ constexpr unsigned kSinglePlainQubits = 20;
 // tile): the kernel multiplies the phase's G factors into one uniform
 // scalar and scales the amplitudes once, after the phase's other ops;
 // PH: a phase on the 8 amplitudes whose register bit j is 1 (the |11>
-// factor of a two-qubit diagonal, its other qubit a control).
+// factor of a two-qubit diagonal, its other qubit a control); P4R: a P4
+// whose pairing coefficients are real on the diagonal and imaginary off
+// it (rxx, ryy), out_r = c_r x_r + i s_r x_(r^3): four real operations per
+// output instead of eight.
 enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5, OP_T2 = 6, OP_S = 7, OP_G = 8,
-                 OP_PH = 9 };
+                 OP_PH = 9, OP_P4R = 10 };
 
 // All program fields are 32/64-bit and uniform across a block, so the
 // kernel reads them with scalar loads into SGPRs.
 struct FOp {                // 64 bytes
     uint32_t type;          // OP_*
-    uint32_t sel;           // M2/A2/X2/T2/PH: register bit j0; M4/P4: 4*j0 + j1 (j0 < j1)
+    uint32_t sel;           // M2/A2/X2/T2/PH: register bit j0; M4/P4/P4R: 4*j0 + j1 (j0 < j1)
     uint32_t pc;            // matrix ops: register indices that satisfy the register controls (0xFFFF: all)
     uint32_t lctrl;         // controls on lane bits (local positions)
     uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)
@@ -95,7 +98,8 @@ struct FOp {                // 64 bytes
 // (1 or 2) holds the ratio T_v / T_0, applied after T_0 by the waves whose
 // operand bit is set -- every table load is uniform.  dmode 0: each lane
 // reads its own variant's table.  M2: 4 complex; M4: 16 complex;
-// P4: (u[r][r], u[r][r^3]) for r = 0..3; A2: (u01, u10); X2: none;
+// P4: (u[r][r], u[r][r^3]) for r = 0..3; P4R: (c_r, s_r) as one complex
+// per r; A2: (u01, u10); X2: none;
 // T2: one complex whose real part is a; S: 16 complex (first diagonal),
 // 4 complex whose real parts are the T2 coefficients of register bits
 // 0..3, 16 complex (second diagonal); G: a D table (entry 0 of variant k is

@@ -102,7 +102,7 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                 }
             }
         } else {
-            CHECK(op.type == OP_M4 || op.type == OP_P4);
+            CHECK(op.type == OP_M4 || op.type == OP_P4 || op.type == OP_P4R);
             const int j0 = (int)op.sel / 4, j1 = (int)op.sel % 4;
             CHECK(j0 < j1 && j1 < kRegBits);
             for (int i = 0; i < kRegs; ++i) {
@@ -115,6 +115,9 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                     if (op.type == OP_M4) {
                         y[r] = 0.0;
                         for (int k = 0; k < 4; ++k) y[r] += cf(P, op.coef, 4 * r + k) * x[k];
+                    } else if (op.type == OP_P4R) {
+                        const C k = cf(P, op.coef, r);
+                        y[r] = k.real() * x[r] + C(0.0, k.imag()) * x[r ^ 3];
                     } else {
                         y[r] = cf(P, op.coef, 2 * r) * x[r] + cf(P, op.coef, 2 * r + 1) * x[r ^ 3];
                     }
