@@ -1065,7 +1065,52 @@ void relocate_diagonals(std::vector<PhasePlan>& plans, const FPass& fp) {
     }
 }
 
-void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
+// Moves the pass's uncontrolled bit flips (x) to its end: a flip of qubit
+// q commutes past a later primitive once that primitive's matrix or table
+// is conjugated by it (index bit q of its rows and columns inverted), so
+// the flips pile up in *flip and the pass applies them as part of its
+// final LDS read-out, at no cost.  A primitive controlled by a pending
+// flip's qubit would need an anti-control: the flip is applied there.
+std::vector<Prim> defer_flips(const std::vector<Prim>& in, uint64_t* flip) {
+    std::vector<Prim> out;
+    out.reserve(in.size());
+    uint64_t F = 0;
+    for (const Prim& p0 : in) {
+        if (p0.type == OP_M2 && p0.ctrl == 0 && p0.m[0] == 0.0 && p0.m[1] == 1.0 && p0.m[2] == 1.0 && p0.m[3] == 0.0) {
+            F ^= 1ull << p0.q0;
+            continue;
+        }
+        for (uint64_t m = p0.ctrl & F; m; m &= m - 1) {
+            const int q = (int)ctz(m);
+            out.push_back(m2(q, 0.0, 1.0, 1.0, 0.0, 0));
+            F &= ~(1ull << q);
+        }
+        Prim p = p0;
+        const int f = (int)((F >> p.q0) & 1u) | (p.q1 >= 0 ? (int)((F >> p.q1) & 1u) << 1 : 0);
+        if (f) {
+            if (p.type == OP_M2) {
+                std::swap(p.m[0], p.m[3]);
+                std::swap(p.m[1], p.m[2]);
+            } else if (p.type == OP_M4) {
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c) p.m[4 * r + c] = p0.m[4 * (r ^ f) + (c ^ f)];
+            } else {  // D: table entry k = a | 2b
+                for (int k = 0; k < 4; ++k) p.m[k] = p0.m[k ^ f];
+            }
+        }
+        out.push_back(p);
+    }
+    *flip = F;
+    return out;
+}
+
+void compile_pass(Program& prog, const PassPlan& pp_in, unsigned dense_min) {
+    uint64_t flip = 0;
+    PassPlan pp;
+    pp.T = pp_in.T;
+    pp.permuted = pp_in.permuted;
+    std::memcpy(pp.sigma, pp_in.sigma, sizeof pp.sigma);
+    pp.prims = defer_flips(pp_in.prims, &flip);
     FPass fp{};
     fp.t = popc(pp.T);
     fp.nthreads = 1u << (fp.t - kRegBits);
@@ -1263,14 +1308,18 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
     };
     fp.direct_first = is_top(prog.phases[fp.phase_begin]) ? 1u : 0u;
     fp.direct_last = is_top(prog.phases[fp.phase_end - 1]) ? 1u : 0u;
-    if (pp.permuted) {
-        // the qubit permutation as a permuted LDS read-out of the tile: the
-        // amplitude at local index e belongs at lam(e) (lam moves local bit
-        // l, index bit tpos[l], to the local bit of index bit sigma[tpos[l]]),
+    if (pp.permuted || flip) {
+        // the deferred flips and the qubit permutation as a permuted LDS
+        // read-out of the tile: the amplitude at local index e belongs at
+        // lam(e ^ f) (f: the flipped local bits; lam moves local bit l,
+        // index bit tpos[l], to the local bit of index bit sigma[tpos[l]]),
         // so the thread that stores local index L reads local index
-        // lam^-1(L), whose swizzled offset is XOR-linear in L's bits
+        // lam^-1(L) ^ f, whose swizzled offset is XOR-linear in L's bits
+        // plus the constant swizzle of f
         int inv[kMaxTile];
         for (uint32_t l = 0; l < fp.t; ++l) inv[local_of(fp.tpos, fp.t, pp.sigma[fp.tpos[l]])] = (int)l;
+        uint32_t f = 0;
+        for (uint64_t m = flip; m; m &= m - 1) f |= 1u << local_of(fp.tpos, fp.t, (int)ctz(m));
         fp.permuted = 1;
         fp.direct_last = 0;
         for (uint32_t k = 0; k + kRegBits < fp.t; ++k) fp.pimg[k] = lds_swizzle(1u << inv[k]);
@@ -1278,7 +1327,7 @@ void compile_pass(Program& prog, const PassPlan& pp, unsigned dense_min) {
             uint32_t e = 0;
             for (int b = 0; b < kRegBits; ++b)
                 if (j >> b & 1) e |= 1u << inv[fp.t - kRegBits + b];
-            fp.pout[j] = lds_swizzle(e);
+            fp.pout[j] = lds_swizzle(e ^ f);
         }
     }
     prog.passes.push_back(fp);
