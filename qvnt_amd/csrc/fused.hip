@@ -419,13 +419,27 @@ __device__ __forceinline__ void dense_phase(cplx<R>* lds, const FPhase& ph, cons
 // Global index of local element 0 of tile `tile_index` (the tiles
 // enumerate the bits outside the tile's qubit set): zeros inserted at the
 // tile's bit positions.
-__device__ __forceinline__ uint64_t tile_base(const FPass& pass, uint64_t tile_index) {
+__device__ __forceinline__ uint64_t tile_base(const uint32_t* tp, uint32_t t, uint64_t tile_index) {
     uint64_t tile = tile_index;
-    for (uint32_t k = 0; k < pass.t; ++k) {
-        const uint64_t lo = tile & ((1ull << pass.tpos[k]) - 1u);
-        tile = ((tile ^ lo) << 1) | lo;
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)kMaxTile; ++k) {
+        if (k < t) {  // uniform
+            const uint64_t lo = tile & ((1ull << tp[k]) - 1u);
+            tile = ((tile ^ lo) << 1) | lo;
+        }
     }
     return tile;
+}
+
+// The pass's local-bit positions from the packed copy (one byte each):
+// four words read at once instead of a dependent scalar load per bit in
+// the loops over them, and four scalar registers instead of thirteen.
+__device__ __forceinline__ void load_tpos(const FPass& pass, uint32_t* tp) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = pass.tpack[i];
+#pragma unroll
+    for (int k = 0; k < kMaxTile; ++k) tp[k] = (w[k / 4] >> (8 * (k % 4))) & 63u;
 }
 
 // This thread's swizzled LDS offset in the direct layout, made opaque at
@@ -440,12 +454,23 @@ __device__ __forceinline__ uint32_t direct_base(uint32_t tid) {
 
 // The tile's final read-out from LDS into the direct layout (register j =
 // local tid + j * nth), through the pass's qubit permutation if it has one.
-template <class R>
+// KARG: the pass is a kernel argument (k_fused): its tables are read bit by
+// bit, in a loop -- fully unrolled, the compiler hoists those loads to the
+// kernel's start and the extra live scalar registers spill.  Passes in
+// global memory (the batched kernels) take the unrolled form: one wide
+// scalar load instead of a dependent load per bit.
+template <class R, bool KARG>
 __device__ __forceinline__ void read_out(cplx<R>* v, const cplx<R>* lds, const FPass& pass, uint32_t tid) {
     if (pass.permuted) {
         uint32_t s = 0;
-        for (uint32_t k = 0; k + kRegBits < pass.t; ++k)
-            if (tid >> k & 1) s ^= pass.pimg[k];
+        if (KARG) {
+            for (uint32_t k = 0; k + kRegBits < pass.t; ++k)
+                if (tid >> k & 1) s ^= pass.pimg[k];
+        } else {
+            // (tid's bits from t-4 up and pimg's entries there are zero)
+#pragma unroll
+            for (uint32_t k = 0; k < (uint32_t)(kMaxTile - kRegBits); ++k) s ^= pass.pimg[k] & (0u - ((tid >> k) & 1u));
+        }
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) v[j] = lds[s ^ pass.pout[j]];
     } else {
@@ -458,7 +483,7 @@ __device__ __forceinline__ void read_out(cplx<R>* v, const cplx<R>* lds, const F
 // The phases of one pass on one tile.  On entry and on exit v holds the
 // thread's amplitudes in the direct layout (register j = local tid + j *
 // nth); `lds` is the tile's swizzled LDS copy (local element e at dswz(e)).
-template <class R>
+template <class R, bool KARG>
 __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPass& pass, const FPhase* __restrict__ phases,
                                             const FOp* __restrict__ ops, const double* __restrict__ coef, uint64_t tile,
                                             uint32_t tid) {
@@ -476,7 +501,7 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
             }
             dense_phase<R>(lds, ph, coef + ph.ucoef, tid);
             __syncthreads();
-            if (p + 1 == p1) read_out<R>(v, lds, pass, tid);
+            if (p + 1 == p1) read_out<R, KARG>(v, lds, pass, tid);
             continue;
         }
         // local base of this thread in the phase: zeros at the register bits
@@ -503,18 +528,19 @@ __device__ __forceinline__ void tile_phases(cplx<R>* v, cplx<R>* lds, const FPas
 #pragma unroll
             for (int i = 0; i < kRegs; ++i) lds[stb ^ ph.sroff[i]] = v[i];
             __syncthreads();
-            if (p + 1 == p1) read_out<R>(v, lds, pass, tid);
+            if (p + 1 == p1) read_out<R, KARG>(v, lds, pass, tid);
         }
     }
 }
 
 // global offset of this thread's local bits 0..t-5
-__device__ __forceinline__ uint64_t thread_offset(const FPass& pass, uint32_t tid) {
+__device__ __forceinline__ uint64_t thread_offset(const uint32_t* tp, uint32_t tid) {
     uint64_t lo_off = 0;
-    for (uint32_t k = 0; k + kRegBits < pass.t; ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << pass.tpos[k];
+    // tid < 2^(t-4): its bits from t-4 up are zero
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)(kMaxTile - kRegBits); ++k) lo_off |= (uint64_t)((tid >> k) & 1u) << tp[k];
     return lo_off;
 }
-
 // every amplitude of the tile is read once and written once per pass:
 // non-temporal stores (streamed through the caches, evicted first)
 template <class R>
@@ -533,12 +559,14 @@ __device__ __forceinline__ void store_direct(cplx<R>* __restrict__ psi, const FP
 // false: the threads run the phases (and their barriers) on their own LDS
 // region but neither read nor write HBM -- a block of the persistent
 // batched kernel whose tile group has no tile (see k_batch_persist).
-template <class R, bool GATED = false>
+template <class R, bool KARG, bool GATED = false>
 __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPass& pass, const FPhase* __restrict__ phases,
                                            const FOp* __restrict__ ops, const double* __restrict__ coef,
                                            uint64_t tile_index, cplx<R>* lds, uint32_t tid, bool active = true) {
-    const uint64_t tile = tile_base(pass, tile_index);
-    const uint64_t base = tile | thread_offset(pass, tid);
+    uint32_t tp[kMaxTile];
+    load_tpos(pass, tp);
+    const uint64_t tile = tile_base(tp, pass.t, tile_index);
+    const uint64_t base = tile | thread_offset(tp, tid);
     cplx<R> v[kRegs];
 #pragma unroll
     for (int j = 0; j < kRegs; ++j) {
@@ -548,7 +576,7 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
             v[j] = cplx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
         }
     }
-    tile_phases<R>(v, lds, pass, phases, ops, coef, tile, tid);
+    tile_phases<R, KARG>(v, lds, pass, phases, ops, coef, tile, tid);
     if (!GATED || active) store_direct(psi, pass, base, v);
 }
 
@@ -566,7 +594,7 @@ template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
                                                const FOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ unsigned char smem[];
-    fused_tile<R>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
+    fused_tile<R, true>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
 }
 
 // The batched form: the tiles of one pass of many registers in one launch
@@ -579,7 +607,7 @@ __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const uns
     extern __shared__ unsigned char smem[];
     const BItem it = items[map[blockIdx.x]];
     const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
-    fused_tile<R>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
+    fused_tile<R, false>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
                   reinterpret_cast<const FOp*>(prog + it.op_off), reinterpret_cast<const double*>(prog + it.coef_off),
                   blockIdx.x - it.block0, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
 }
@@ -620,12 +648,12 @@ __global__ QVNT_FUSED_BOUNDS void k_batch_persist(cplx<R>* __restrict__ arena, c
             if (tbits >= 6) {
                 const uint32_t group = __builtin_amdgcn_readfirstlane(threadIdx.x >> tbits);
                 const uint32_t tile = un.tile0 + group;
-                fused_tile<R, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
                                     reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
             } else {
                 const uint32_t group = threadIdx.x >> tbits;
                 const uint32_t tile = un.tile0 + group;
-                fused_tile<R, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
                                     reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
             }
             // the next unit's first LDS writes may hit elements other

@@ -1117,6 +1117,7 @@ void compile_pass(Program& prog, const PassPlan& pp_in, unsigned dense_min) {
     {
         unsigned k = 0;
         for (uint64_t m = pp.T; m; m &= m - 1) fp.tpos[k++] = ctz(m);
+        for (k = 0; k < fp.t; ++k) fp.tpack[k / 4] |= fp.tpos[k] << (8 * (k % 4));
     }
     for (int j = 0; j < kRegs; ++j) {
         uint64_t off = 0;

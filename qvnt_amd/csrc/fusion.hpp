@@ -127,6 +127,9 @@ struct FPass {
     uint32_t permuted;
     uint32_t pimg[kMaxTile - kRegBits];
     uint32_t pout[kRegs];
+    // tpos again, one byte each (byte k of the 16 = tpos[k], 0 past t):
+    // four scalar registers for the kernel's loops over the local bits
+    uint32_t tpack[4];
 };
 
 // One register's pass in a batched launch: byte offsets into the program

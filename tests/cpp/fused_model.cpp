@@ -159,6 +159,7 @@ static void run_pass(C* psi, uint64_t len, const Program& P, const FPass& pass) 
     const uint32_t t = pass.t, nth = pass.nthreads;
     CHECK(t >= 4 && t <= (uint32_t)kMaxTile && nth == (1u << (t - kRegBits)));
     for (uint32_t k = 1; k < t; ++k) CHECK(pass.tpos[k - 1] < pass.tpos[k]);
+    for (uint32_t k = 0; k < 16; ++k) CHECK(((pass.tpack[k / 4] >> (8 * (k % 4))) & 255u) == (k < t ? pass.tpos[k] : 0u));
     CHECK((1ull << pass.tpos[t - 1]) < len);
     CHECK(pass.phase_begin < pass.phase_end && pass.phase_end <= P.phases.size());
     std::vector<C> lds(1u << t), V(nth * kRegs);
@@ -329,7 +330,7 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
                 u.phase_end != v.phase_end || u.direct_first != v.direct_first || u.direct_last != v.direct_last ||
                 std::memcmp(u.tpos, v.tpos, sizeof u.tpos) != 0 || std::memcmp(u.hoff, v.hoff, sizeof u.hoff) != 0 ||
                 u.permuted != v.permuted || std::memcmp(u.pimg, v.pimg, sizeof u.pimg) != 0 ||
-                std::memcmp(u.pout, v.pout, sizeof u.pout) != 0)
+                std::memcmp(u.pout, v.pout, sizeof u.pout) != 0 || std::memcmp(u.tpack, v.tpack, sizeof u.tpack) != 0)
                 return false;
         }
         if (std::memcmp(p.phases.data(), q.phases.data(), p.phases.size() * sizeof(FPhase)) != 0 ||
