@@ -125,8 +125,7 @@ __device__ __forceinline__ void apply_t2(cplx<R>* v, R a, uint32_t pc) {
 }
 
 template <class R>
-__device__ __forceinline__ void apply_t2_sel(cplx<R>* v, const double* __restrict__ c, uint32_t sel, uint32_t pc) {
-    const R a = (R)c[0];
+__device__ __forceinline__ void apply_t2_sel(cplx<R>* v, R a, uint32_t sel, uint32_t pc) {
     if (pc == 0xFFFFu) {
         switch (sel) {
         case 0: apply_t2<0, false>(v, a, 0u); break;
@@ -190,8 +189,7 @@ __device__ __forceinline__ void apply_d(cplx<R>* v, const double* __restrict__ c
 
 // a phase on the amplitudes whose register bit J is 1 (fusion.hpp: OP_PH)
 template <int J, class R>
-__device__ __forceinline__ void apply_ph(cplx<R>* v, const double* __restrict__ c) {
-    const cplx<R> g = ld_coef<R>(c, 0);
+__device__ __forceinline__ void apply_ph(cplx<R>* v, const cplx<R> g) {
 #pragma unroll
     for (int i = 0; i < kRegs; ++i)
         if (i & (1 << J)) v[i] = mul_c(g, v[i]);
@@ -210,6 +208,12 @@ __device__ __forceinline__ void apply_s(cplx<R>* v, const double* __restrict__ c
     apply_d(v, c + 2 * (kRegs + kRegBits));
 }
 
+
+// T2's and PH's coefficient, carried in the op (fusion.hpp: FOp::ga, gb)
+template <class R>
+__device__ __forceinline__ cplx<R> inline_coef(const FOp& op) {
+    return cplx<R>{(R)__longlong_as_double((long long)op.ga), (R)__longlong_as_double((long long)op.gb)};
+}
 
 template <class R>
 __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FOp* __restrict__ ops,
@@ -251,13 +255,13 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
             apply_s(v, c);
         } else if (op.type == OP_PH) {
             switch (op.sel) {
-            case 0: apply_ph<0>(v, c); break;
-            case 1: apply_ph<1>(v, c); break;
-            case 2: apply_ph<2>(v, c); break;
-            default: apply_ph<3>(v, c); break;
+            case 0: apply_ph<0>(v, inline_coef<R>(op)); break;
+            case 1: apply_ph<1>(v, inline_coef<R>(op)); break;
+            case 2: apply_ph<2>(v, inline_coef<R>(op)); break;
+            default: apply_ph<3>(v, inline_coef<R>(op)); break;
             }
         } else if (op.type == OP_T2) {
-            apply_t2_sel(v, c, op.sel, op.pc);
+            apply_t2_sel(v, inline_coef<R>(op).re, op.sel, op.pc);
         } else if (op.type == OP_M2) {
             if (op.pc == 0xFFFFu) {
                 switch (op.sel) {

@@ -894,7 +894,7 @@ uint32_t coef_complex(const FOp& op) {
     case OP_A2: return 2;
     case OP_X2: return 0;
     case OP_S: return 2 * kRegs + kRegBits;
-    default: return 1;  // T2, PH
+    default: return 0;  // T2, PH: inline in the op
     }
 }
 
@@ -986,6 +986,14 @@ void finish_phase(Program& prog, uint32_t begin, size_t coef_begin, const FPhase
         phase_on_bit(prog, prog.ops[o]);
     }
     prog.ops.insert(prog.ops.end(), scalars.begin(), scalars.end());
+    // T2 and PH carry their one coefficient in the op itself
+    for (size_t o = begin; o < prog.ops.size(); ++o) {
+        FOp& op = prog.ops[o];
+        if (op.type != OP_T2 && op.type != OP_PH) continue;
+        const double re = prog.coef[op.coef], im = prog.coef[op.coef + 1];
+        std::memcpy(&op.ga, &re, sizeof re);
+        std::memcpy(&op.gb, &im, sizeof im);
+    }
     // keep only the coefficients the phase's ops read (merged, folded and
     // rewritten tables leave unused blocks behind)
     std::vector<double> kept;

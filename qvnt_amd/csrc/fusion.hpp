@@ -87,7 +87,9 @@ struct FOp {                // 64 bytes
     uint32_t coef;          // offset (in doubles) into the coefficient array
     uint32_t dmode;         // D: 0 per-lane variant table, 1 ratio table (see below)
     uint64_t gctrl;         // controls on bits outside the tile (global positions)
-    uint64_t ga, gb;        // D: operand a / b when it lies outside the tile
+    uint64_t ga, gb;        // D: operand a / b when it lies outside the tile; T2, PH: the op's one
+                            // coefficient inline, real part in ga and imaginary part in gb
+                            // (double bit patterns), so the kernel needs no dependent load for it
     uint64_t pad2;
 };
 // D coefficients: 4 variants (k = [a is a non-register 1] | 2 [b is a
@@ -100,10 +102,10 @@ struct FOp {                // 64 bytes
 // reads its own variant's table.  M2: 4 complex; M4: 16 complex;
 // P4: (u[r][r], u[r][r^3]) for r = 0..3; P4R: (c_r, s_r) as one complex
 // per r; A2: (u01, u10); X2: none;
-// T2: one complex whose real part is a; S: 16 complex (first diagonal),
+// T2: a (inline in the op); S: 16 complex (first diagonal),
 // 4 complex whose real parts are the T2 coefficients of register bits
 // 0..3, 16 complex (second diagonal); G: a D table (entry 0 of variant k is
-// its factor); PH: one complex.
+// its factor); PH: one complex (inline in the op).
 
 struct FPhase {
     uint32_t op_begin, op_end;

@@ -33,7 +33,16 @@ static C cf(const Program& P, uint32_t off, int k) {
     return C(P.coef[off + 2 * k], P.coef[off + 2 * k + 1]);
 }
 
+// T2's and PH's coefficient, carried in the op (FOp::ga: real, gb: imaginary)
+static C inline_coef(const FOp& op) {
+    double re, im;
+    std::memcpy(&re, &op.ga, sizeof re);
+    std::memcpy(&im, &op.gb, sizeof im);
+    return C(re, im);
+}
+
 // One register phase of one thread: the ops of fused.hip's run_phase.
+
 static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uint64_t tile) {
     CHECK(ph.op_begin <= ph.op_end && ph.op_end <= P.ops.size());
     // the kernel runs a phase's G ops after all others: they must come last
@@ -62,9 +71,10 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                 for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, k * kRegs + i) * v[i];
             }
         } else if (op.type == OP_PH) {
-            CHECK(op.sel < (uint32_t)kRegBits && !op.la && !op.lb && !op.ga && !op.gb);
+            CHECK(op.sel < (uint32_t)kRegBits && !op.la && !op.lb);
+            const C g = inline_coef(op);
             for (int i = 0; i < kRegs; ++i)
-                if ((i >> op.sel) & 1) v[i] = cf(P, op.coef, 0) * v[i];
+                if ((i >> op.sel) & 1) v[i] = g * v[i];
         } else if (op.type == OP_S) {
             for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
             for (int j = 0; j < kRegBits; ++j) {
@@ -92,8 +102,8 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                     v[i] = cf(P, op.coef, 0) * x1;
                     v[i | (1 << J)] = cf(P, op.coef, 1) * x0;
                 } else if (op.type == OP_T2) {
-                    const double a = cf(P, op.coef, 0).real();
-                    CHECK(cf(P, op.coef, 0).imag() == 0.0);
+                    const double a = inline_coef(op).real();
+                    CHECK(inline_coef(op).imag() == 0.0);
                     v[i] = x0 - a * x1;
                     v[i | (1 << J)] = x1 + a * x0;
                 } else {
