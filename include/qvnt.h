@@ -399,7 +399,7 @@ typedef struct {
     uint32_t rounds;          /* sequential rounds: the most passes of any register */
     uint32_t launches;        /* kernel launches per run */
     uint64_t passes;          /* register passes per run */
-    uint64_t tiles;           /* tiles (workgroups) per run */
+    uint64_t tiles;           /* tiles per run (k_batch runs two per workgroup in FP64) */
     uint64_t single_ops;      /* reference SingleOps of all circuits */
     double bytes;             /* algorithmic HBM bytes per run: 2 * amp * 2^n per pass */
     uint64_t phases;          /* phases per run (all passes) */

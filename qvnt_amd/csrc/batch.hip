@@ -255,7 +255,9 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
                 it.block0 = (uint32_t)map.size();
                 const uint64_t tiles = 1ull << (b->q[k] - fp.t);
                 if (map.size() + tiles >= (1ull << 31)) return fail(QVNT_E_ARG, "too many tiles in one round");
-                for (uint64_t j = 0; j < tiles; ++j) map.push_back((uint32_t)items.size());
+                it.tiles = (uint32_t)tiles;
+                const uint32_t per_block = qvnt::fusion::batch_tiles(b->prec == QVNT_F64);
+                for (uint64_t j = 0; j < tiles; j += per_block) map.push_back((uint32_t)items.size());
                 items.push_back(it);
                 const uint64_t per_unit = 1ull << (S.tmax - fp.t);
                 for (uint64_t j = 0; j < tiles; j += per_unit)

@@ -602,18 +602,29 @@ __global__ QVNT_FUSED_BOUNDS void k_fused(cplx<R>* __restrict__ psi, FPass pass,
 }
 
 // The batched form: the tiles of one pass of many registers in one launch
-// (every item's pass has the launch's t).  Block b runs tile
-// b - items[map[b]].block0 of item map[b]; registers, passes and programs
-// are addressed by offsets from the two base pointers.
+// (every item's pass has the launch's t).  Block b runs batch_tiles tiles
+// of item map[b], from tile (b - items[map[b]].block0) * batch_tiles, one
+// after the other; registers, passes and programs are addressed by offsets
+// from the two base pointers.
 template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
                                                const BItem* __restrict__ items, const uint32_t* __restrict__ map) {
     extern __shared__ unsigned char smem[];
     const BItem it = items[map[blockIdx.x]];
     const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
-    fused_tile<R, false>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
-                  reinterpret_cast<const FOp*>(prog + it.op_off), reinterpret_cast<const double*>(prog + it.coef_off),
-                  blockIdx.x - it.block0, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
+    constexpr uint32_t per_block = batch_tiles(sizeof(R) == 8);
+    const uint32_t first = (blockIdx.x - it.block0) * per_block;
+    for (uint32_t k = 0; k < per_block; ++k) {
+        const uint32_t tile = first + k;
+        if (tile >= it.tiles) break;  // uniform
+        // the tile's first LDS writes may hit elements other threads still
+        // read in the previous tile's last phase
+        if (k) __syncthreads();
+        fused_tile<R, false>(arena + it.psi_off, pass, reinterpret_cast<const FPhase*>(prog + it.phase_off),
+                             reinterpret_cast<const FOp*>(prog + it.op_off),
+                             reinterpret_cast<const double*>(prog + it.coef_off), tile, reinterpret_cast<cplx<R>*>(smem),
+                             threadIdx.x);
+    }
 }
 
 // The batched form as one persistent launch (cooperative: every block is

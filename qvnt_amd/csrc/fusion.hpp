@@ -143,8 +143,16 @@ struct BItem {              // 32 bytes
     uint32_t op_off;        // bytes: its FOp array
     uint32_t coef_off;      // bytes: its coefficients
     uint32_t block0;        // first block of the item in its launch
-    uint32_t pad;
+    uint32_t tiles;         // tiles of the pass (k_batch: batch_tiles per block)
 };
+
+// Tiles one block of k_batch runs, one after the other.  A block's tiles
+// share its register pass, so the program's headers and coefficients are
+// in the scalar cache after the first; that pays in FP64 (two waves per
+// SIMD, little latency hidden: 4096-register batch 48.9 ms per run with one
+// tile per block, 47.2 with two or four) but not in FP32 (four waves per
+// SIMD: 27.1 ms with one, 32.1 with two).
+constexpr uint32_t batch_tiles(bool fp64) { return fp64 ? 2u : 1u; }
 
 // LDS swizzle: bits 4..7 of the local index are XORed into bits 0..3, so
 // 16 lanes that differ in bits 4..7 hit 16 different 16-byte bank groups.
