@@ -548,9 +548,10 @@ def batch_leg(registers, n_hi, prec, tile_cap, dense, steps, warmup, check, plai
             same = all(np.array_equal(b.get_state(k), first[k]) for k in range(sample))
             ms_persist = b.time(steps, graph="persistent")
             persist = {"ms_per_run": round(ms_persist, 3), "blocks": info["persist_blocks"],
-                       "value": round(single / (ms_persist * 1e-3), 1), "equal_to_rounds": same}
-            log("persistent launch (%d blocks): %.2f ms/run; equal to the rounds on %d registers: %s" % (
-                info["persist_blocks"], ms_persist, sample, same))
+                       "value": round(single / (ms_persist * 1e-3), 1), "equal_to_rounds": same,
+                       "launch": b.persist_form}
+            log("persistent launch (%d blocks, %s): %.2f ms/run; equal to the rounds on %d registers: %s" % (
+                info["persist_blocks"], b.persist_form, ms_persist, sample, same))
     gbs = info["bytes"] / (ms_graph * 1e-3) / 1e9
     value = single / (ms_graph * 1e-3)
     tflops = info["dense_flops"] / (ms_graph * 1e-3) / 1e12

@@ -435,10 +435,18 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
  * as separate launches (one per round and tile size); 1: the same captured
  * once (at the first such call after a compile) into a hipGraph and
  * replayed; 2: one persistent cooperative launch whose resident blocks walk
- * all rounds with a grid barrier between them (not captured: it is one
- * dispatch already); fails with QVNT_E_HIP where info->persist_blocks was 0.
- * All modes give the same amplitudes bit for bit. */
+ * all rounds with a grid barrier between them, captured once into a
+ * hipGraph and replayed (the first such call checks that the captured
+ * launch ran cooperatively; if not, it and every later call launch the
+ * kernel directly with hipLaunchCooperativeKernel, see
+ * qvnt_batch_persist_form); fails with QVNT_E_HIP where
+ * info->persist_blocks was 0.  All modes give the same amplitudes bit for
+ * bit. */
 int qvnt_batch_run(qvnt_batch* b, int graph);
+/* How mode 2 of qvnt_batch_run runs (no reference counterpart): *form = 0
+ * not run since the last compile, 1 the captured hipGraph, 2 direct
+ * cooperative launches. */
+int qvnt_batch_persist_form(const qvnt_batch* b, int* form);
 /* Runs `runs` times between two device events (graph as qvnt_batch_run);
  * *ms = milliseconds per run. */
 int qvnt_batch_time(qvnt_batch* b, uint32_t runs, int graph, double* ms);

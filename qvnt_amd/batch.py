@@ -31,6 +31,7 @@ def _declare():
         "qvnt_batch_reset": ([vp], i),
         "qvnt_batch_compile": ([vp, P(vp), u32, u32, P(BatchInfo)], i),
         "qvnt_batch_run": ([vp, i], i),
+        "qvnt_batch_persist_form": ([vp, P(i)], i),
         "qvnt_batch_time": ([vp, u32, i, P(d)], i),
         "qvnt_batch_sync": ([vp], i),
         "qvnt_batch_get_state": ([vp, u32, vp, ctypes.c_size_t], i),
@@ -94,8 +95,18 @@ class Batch:
     def run(self, graph=False):
         """One application of every compiled circuit: graph False -- the
         rounds as launches; True -- the same as one replayed hipGraph;
-        "persistent" -- one persistent cooperative launch over all rounds."""
+        "persistent" -- one persistent cooperative launch over all rounds,
+        captured into a hipGraph and replayed (see persist_form)."""
         check(lib.qvnt_batch_run(self._h, self._mode(graph)))
+
+    @property
+    def persist_form(self):
+        """How run(graph="persistent") runs: None (not yet since the last
+        compile), "graph" (the cooperative launch captured into a hipGraph)
+        or "direct" (hipLaunchCooperativeKernel)."""
+        f = ctypes.c_int()
+        check(lib.qvnt_batch_persist_form(self._h, ctypes.byref(f)))
+        return {0: None, 1: "graph", 2: "direct"}[f.value]
 
     def time(self, runs, graph=False):
         ms = ctypes.c_double()

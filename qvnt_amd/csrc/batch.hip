@@ -105,7 +105,7 @@ struct Schedule {
     // the persistent form: work units of every round (k_batch_persist)
     uint32_t tmax = 0;             // largest tile (the block holds 2^(tmax-4) threads)
     uint32_t persist_blocks = 0;   // resident blocks of the cooperative launch (0: not available)
-    size_t p_items_off = 0, p_units_off = 0, p_rounds_off = 0;   // bytes into dpersist
+    size_t p_items_off = 0, p_units_off = 0, p_rounds_off = 0, p_status_off = 0;   // bytes into dpersist
     void* dpersist = nullptr;
     void release() {
         if (dprog) (void)hipFree(dprog);
@@ -135,14 +135,23 @@ struct qvnt_batch {
     uint32_t caps[kMaxQubits + 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the persistent launch captured into a graph; persist_form: 0 not run
+    // yet, 1 the graph, 2 direct cooperative launches (the capture failed or
+    // its launch was not cooperative)
+    hipGraph_t pgraph = nullptr;
+    hipGraphExec_t pexec = nullptr;
+    int persist_form = 0;
     qvnt_reg view;                 // non-owning register on one batch member (readback, sampling)
 
     size_t amp() const { return prec == QVNT_F64 ? 16 : 8; }
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        if (pexec) (void)hipGraphExecDestroy(pexec);
+        if (pgraph) (void)hipGraphDestroy(pgraph);
+        exec = pexec = nullptr;
+        graph = pgraph = nullptr;
+        persist_form = 0;
     }
 };
 
@@ -293,7 +302,9 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
     S.p_items_off = 0;
     S.p_units_off = a256(p_items.size() * sizeof(BItem));
     S.p_rounds_off = a256(S.p_units_off + p_units.size() * sizeof(PUnit));
-    std::vector<unsigned char> phost(a256(S.p_rounds_off + p_rounds.size() * sizeof(uint32_t)));
+    S.p_status_off = a256(S.p_rounds_off + p_rounds.size() * sizeof(uint32_t));
+    // (the status word starts at zero: phost is zeroed)
+    std::vector<unsigned char> phost(a256(S.p_status_off + sizeof(uint32_t)));
     if (!p_items.empty()) std::memcpy(phost.data() + S.p_items_off, p_items.data(), p_items.size() * sizeof(BItem));
     if (!p_units.empty()) std::memcpy(phost.data() + S.p_units_off, p_units.data(), p_units.size() * sizeof(PUnit));
     std::memcpy(phost.data() + S.p_rounds_off, p_rounds.data(), p_rounds.size() * sizeof(uint32_t));
@@ -321,14 +332,57 @@ int enqueue(qvnt_batch* b, const Schedule& S) {
     return QVNT_OK;
 }
 
-// The whole schedule as one cooperative launch of the persistent kernel.
-int enqueue_persist(qvnt_batch* b, const Schedule& S) {
+// The whole schedule as one cooperative launch of the persistent kernel
+// (graph non-null: build that launch as a graph instead).
+int enqueue_persist(qvnt_batch* b, const Schedule& S, hipGraph_t* graph = nullptr) {
     if (!S.persist_blocks) return fail(QVNT_E_HIP, "the persistent batched form is not available on this device");
-    const unsigned char* P = static_cast<const unsigned char*>(S.dpersist);
+    unsigned char* P = static_cast<unsigned char*>(S.dpersist);
     return qvnt::launch_batch_persist(b->stream, b->prec, b->arena, S.dprog, reinterpret_cast<const BItem*>(P + S.p_items_off),
                                       reinterpret_cast<const PUnit*>(P + S.p_units_off),
                                       reinterpret_cast<const uint32_t*>(P + S.p_rounds_off), S.rounds, S.tmax,
-                                      S.persist_blocks);
+                                      S.persist_blocks, reinterpret_cast<uint32_t*>(P + S.p_status_off), graph);
+}
+
+// Mode 2 of qvnt_batch_run.  The first call captures the cooperative
+// launch into a graph, runs it and reads the status word: a launch the
+// runtime did not make cooperative leaves every register untouched and the
+// status set, so the graph is dropped, the status cleared and the run made
+// with a direct cooperative launch, as every later one is.
+int run_persist(qvnt_batch* b) {
+    const Schedule& S = b->sched;
+    if (b->persist_form == 1) {
+        QVNT_HIP(hipGraphLaunch(b->pexec, b->stream));
+        return QVNT_OK;
+    }
+    if (b->persist_form == 0) {
+        b->persist_form = 2;
+        hipGraph_t g = nullptr;
+        if (enqueue_persist(b, S, &g) == QVNT_OK && hipGraphInstantiate(&b->pexec, g, nullptr, nullptr, 0) == hipSuccess) {
+            b->pgraph = g;
+            uint32_t status = 0;
+            unsigned char* P = static_cast<unsigned char*>(S.dpersist);
+            QVNT_HIP(hipGraphLaunch(b->pexec, b->stream));
+            QVNT_HIP(hipMemcpyAsync(&status, P + S.p_status_off, sizeof status, hipMemcpyDeviceToHost, b->stream));
+            QVNT_HIP(hipStreamSynchronize(b->stream));
+            if (status == 0) {
+                b->persist_form = 1;
+                return QVNT_OK;
+            }
+            status = 0;
+            QVNT_HIP(hipMemcpyAsync(P + S.p_status_off, &status, sizeof status, hipMemcpyHostToDevice, b->stream));
+            QVNT_HIP(hipStreamSynchronize(b->stream));
+            (void)hipGraphExecDestroy(b->pexec);
+            (void)hipGraphDestroy(b->pgraph);
+            b->pexec = nullptr;
+            b->pgraph = nullptr;
+        } else {
+            if (g) (void)hipGraphDestroy(g);
+            b->pexec = nullptr;
+            (void)hipGetLastError();
+            qvnt::clear_error();
+        }
+    }
+    return enqueue_persist(b, S);
 }
 
 int check_batch(const qvnt_batch* b) {
@@ -561,10 +615,9 @@ int qvnt_batch_compile(qvnt_batch* b, const qvnt_op* const* ops, uint32_t max_qu
 int qvnt_batch_run(qvnt_batch* b, int use_graph) {
     if (int rc = check_batch(b)) return rc;
     if (b->sched.launches.empty()) return QVNT_OK;
-    // one persistent launch; never captured: a cooperative launch is
-    // already one dispatch, and stream capture of one is not part of HIP's
-    // documented capture support
-    if (use_graph == 2) return enqueue_persist(b, b->sched);
+    // one persistent launch, replayed from a graph where its captured
+    // launch is cooperative (run_persist)
+    if (use_graph == 2) return run_persist(b);
     if (!use_graph) return enqueue(b, b->sched);
     if (!b->exec) {
         QVNT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -580,6 +633,12 @@ int qvnt_batch_run(qvnt_batch* b, int use_graph) {
         QVNT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     QVNT_HIP(hipGraphLaunch(b->exec, b->stream));
+    return QVNT_OK;
+}
+
+int qvnt_batch_persist_form(const qvnt_batch* b, int* form) {
+    if (!b || !form) return fail(QVNT_E_ARG, "null argument");
+    *form = b->persist_form;
     return QVNT_OK;
 }
 

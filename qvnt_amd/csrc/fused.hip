@@ -643,12 +643,22 @@ __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const uns
 // The launch bounds are k_fused's: they leave the register allocation at
 // what the kernel uses, which is what the runtime's occupancy query (and so
 // the cooperative grid size) counts.
+//
+// A launch that is not cooperative (a graph node whose cooperative
+// attribute the runtime did not honour) has no grid barrier: every block
+// sees that alike, block 0 sets *status to 1, and all return before any
+// barrier or HBM access.
 template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_batch_persist(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
                                                   const BItem* __restrict__ items, const PUnit* __restrict__ units,
-                                                  const uint32_t* __restrict__ round_begin, uint32_t rounds) {
+                                                  const uint32_t* __restrict__ round_begin, uint32_t rounds,
+                                                  uint32_t* status) {
     extern __shared__ unsigned char smem[];
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    if (!grid.is_valid()) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *status = 1u;
+        return;
+    }
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t u_end = round_begin[r + 1];
         for (uint32_t u = round_begin[r] + blockIdx.x; u < u_end; u += gridDim.x) {
@@ -717,15 +727,32 @@ int batch_persist_grid(qvnt_prec prec, uint32_t tmax, uint32_t* blocks) {
 
 int launch_batch_persist(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const BItem* items,
                          const PUnit* units, const uint32_t* round_begin, uint32_t rounds, uint32_t tmax,
-                         uint32_t blocks) {
+                         uint32_t blocks, uint32_t* status, hipGraph_t* graph) {
     const size_t amp = prec == QVNT_F64 ? 16 : 8;
     const dim3 grid(blocks), block(1u << (tmax - kRegBits));
     const size_t lds = ((size_t)1 << tmax) * amp;
     const unsigned char* p = static_cast<const unsigned char*>(prog);
-    void* args[] = {&arena, &p, &items, &units, &round_begin, &rounds};
-    const void* fn = prec == QVNT_F64 ? reinterpret_cast<const void*>(k_batch_persist<double>)
-                                      : reinterpret_cast<const void*>(k_batch_persist<float>);
-    QVNT_HIP(hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream));
+    void* args[] = {&arena, &p, &items, &units, &round_begin, &rounds, &status};
+    void* fn = prec == QVNT_F64 ? reinterpret_cast<void*>(k_batch_persist<double>)
+                                : reinterpret_cast<void*>(k_batch_persist<float>);
+    if (!graph) {
+        QVNT_HIP(hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream));
+        return QVNT_OK;
+    }
+    // the cooperative launch captured from the stream into a graph (on
+    // ROCm 7.2 a captured cooperative launch stays cooperative, while a
+    // kernel node built by hand with the cooperative attribute does not;
+    // the kernel's own check covers either)
+    QVNT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(stream, &g);
+    if (e != hipSuccess || end != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return fail(QVNT_E_HIP, "capturing the persistent batched launch");
+    }
+    *graph = g;
     return QVNT_OK;
 }
 

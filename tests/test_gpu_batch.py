@@ -118,11 +118,15 @@ def test_persistent_launch_equals_rounds(prec, cap, dense):
         b.run()
         b.run()
         want = [b.get_state(k) for k in range(len(regs))]
+        assert b.persist_form is None
         for rep in range(2):
             b.reset()
             b.run(graph="persistent")
             b.run(graph="persistent")
             b.sync()
+            print("persistent launch form:", b.persist_form)
+            # a captured cooperative launch stays cooperative on this runtime
+            assert b.persist_form == "graph"
             for k, (n, _) in enumerate(regs):
                 assert np.array_equal(b.get_state(k), want[k]), "run %d, register %d (%d qubits)" % (rep, k, n)
 
