@@ -260,3 +260,57 @@ def test_tfim_ring_round_trip_largest_register(n, prec):
         outs = r.sample_mask((1 << n) - 1, [qvnt_amd.uniform(8, n, k) for k in range(8)])
         assert all(int(x) == 0 for x in outs)
         assert abs(r.get_absolute() - 1.0) < tol
+
+
+def permutation_circuit(g, n, count):
+    """x (some controlled), swap and iswap (some daggered, some controlled):
+    every amplitude is moved, or moved and multiplied by +-i, so the plain
+    and the fused form must agree bit for bit."""
+    out = op.id()
+    for _ in range(count):
+        kind = int(g.integers(6))
+        a = int(g.integers(n))
+        b = (a + 1 + int(g.integers(n - 1))) % n
+        c = (b + 1 + int(g.integers(n - 2))) % n
+        if c == a:
+            c = (c + 1) % n
+        if kind == 0:
+            o = op.x(1 << a)
+        elif kind == 1:
+            o = op.x(1 << a).c(1 << b)
+        elif kind == 2:
+            o = op.swap((1 << a) | (1 << b))
+        elif kind == 3:
+            o = op.i_swap((1 << a) | (1 << b))
+            if g.integers(2):
+                o = o.dgr()
+        elif kind == 4 and c != b:
+            o = op.swap((1 << a) | (1 << b)).c(1 << c)
+        else:
+            o = op.i_swap((1 << a) | (1 << b))
+        out *= o
+    return out
+
+
+@pytest.mark.parametrize("n,tq,prec", [(10, 10, "f64"), (16, 10, "f64"), (20, 10, "f64"), (20, 12, "f32"),
+                                       (24, 11, "f64")])
+def test_relabeled_permutations_bit_exact(n, tq, prec):
+    """Uncontrolled swaps and iswaps are relabelings in the fused form and
+    uncontrolled flips move to the end of their pass; each pass then
+    permutes its tile's qubits in the LDS read-out, and passes without
+    gates bring the last qubits home.  A circuit of permutations checks
+    all of it exactly, against the plain form (one kernel per SingleOp),
+    from a random state."""
+    g = np.random.default_rng(7100 + n + tq)
+    circ = permutation_circuit(g, n, 300)
+    dt = np.complex128 if prec == "f64" else np.complex64
+    psi = (g.standard_normal(1 << n) + 1j * g.standard_normal(1 << n)).astype(dt)
+    with QReg(n, prec=prec) as r:
+        r.set_state(psi)
+        r.apply(circ)
+        want = r.get_state()[: 1 << n]
+        r.set_state(psi)
+        passes = r.apply_fused(circ, max_qubits=tq)
+        got = r.get_state()[: 1 << n]
+    say("permutations n=%d tq=%d %s: %d gates, %d passes" % (n, tq, prec, len(circ), passes))
+    assert np.array_equal(got, want)
