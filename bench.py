@@ -452,6 +452,15 @@ def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0
         totals_plain = {r["id"]: r["total"] for r in res}
     res = closed_loop(True)
     out["serial_fused"] = stats(res, "serial fused")
+    # the overlap bound: requests of 2^27+ amplitudes run one at a time in
+    # the server, so its makespan is at least their serial service time
+    service = {r["id"]: r["t_done"] - r["t_submit"] for r in res}
+    big = sum(t for k, t in service.items() if qs[k] >= 27)
+    out["serial_fused"]["service_s_27q_plus"] = round(big, 3)
+    out["serial_fused"]["service_s_all"] = round(sum(service.values()), 3)
+    out["overlap_bound"] = round(sum(service.values()) / big, 3) if big else None
+    log("serial fused: requests of 27+ qubits take %.3f of %.3f s of service; overlap can raise the rate at most %.2fx" % (
+        big, sum(service.values()), out["overlap_bound"] or 0.0))
     outcomes = {r["id"]: (r["outcomes"], r["total"]) for r in res}
     if totals_plain:
         out["max_total_diff_fused_vs_plain"] = max(abs(outcomes[k][1] - totals_plain[k]) for k in totals_plain)
