@@ -2,7 +2,8 @@
 //
 // The host compiler (fusion.cpp) turns the queue into passes; this kernel
 // runs one pass.  One block per tile of 2^t amplitudes (the tile's t qubits
-// are the pass's qubit set).  The tile is read once from HBM, every phase
+// are the pass's qubit set; blocks take tiles in XCD order, xcd_order).
+// The tile is read once from HBM, every phase
 // moves its 16 amplitudes per thread LDS -> registers -> LDS (skipped when
 // the phase's register qubits are the tile's top four, the layout of the
 // HBM access itself), and the tile is written back once.  HBM traffic is
@@ -594,26 +595,40 @@ __device__ __forceinline__ void fused_tile(cplx<R>* __restrict__ psi, const FPas
 #define QVNT_FUSED_BOUNDS __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(1, 2)))
 #endif
 
+// XCD-aware block order.  The dispatcher hands workgroups to the eight XCDs
+// round robin (block b to XCD b mod 8), and each XCD has its own L2 and
+// address translation.  With G = 8q + r blocks, block b < 8q runs work
+// item (b mod 8) q + b / 8, so XCD x walks the contiguous range
+// [x q, (x + 1) q): its concurrently running blocks touch neighbouring
+// lines and pages.  The last r blocks keep their index.  A bijection on
+// [0, G).
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t G) {
+    const uint32_t q = G >> 3;
+    return b < (q << 3) ? (b & 7u) * q + (b >> 3) : b;
+}
+
 template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_fused(cplx<R>* __restrict__ psi, FPass pass, const FPhase* __restrict__ phases,
                                                const FOp* __restrict__ ops, const double* __restrict__ coef) {
     extern __shared__ unsigned char smem[];
-    fused_tile<R, true>(psi, pass, phases, ops, coef, blockIdx.x, reinterpret_cast<cplx<R>*>(smem), threadIdx.x);
+    fused_tile<R, true>(psi, pass, phases, ops, coef, xcd_order(blockIdx.x, gridDim.x), reinterpret_cast<cplx<R>*>(smem),
+                        threadIdx.x);
 }
 
 // The batched form: the tiles of one pass of many registers in one launch
-// (every item's pass has the launch's t).  Block b runs batch_tiles tiles
-// of item map[b], from tile (b - items[map[b]].block0) * batch_tiles, one
-// after the other; registers, passes and programs are addressed by offsets
-// from the two base pointers.
+// (every item's pass has the launch's t).  Block b (in XCD order,
+// xcd_order) runs batch_tiles tiles of item map[b], from tile
+// (b - items[map[b]].block0) * batch_tiles, one after the other; registers,
+// passes and programs are addressed by offsets from the two base pointers.
 template <class R>
 __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
                                                const BItem* __restrict__ items, const uint32_t* __restrict__ map) {
     extern __shared__ unsigned char smem[];
-    const BItem it = items[map[blockIdx.x]];
+    const uint32_t b = xcd_order(blockIdx.x, gridDim.x);
+    const BItem it = items[map[b]];
     const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
     constexpr uint32_t per_block = batch_tiles(sizeof(R) == 8);
-    const uint32_t first = (blockIdx.x - it.block0) * per_block;
+    const uint32_t first = (b - it.block0) * per_block;
     for (uint32_t k = 0; k < per_block; ++k) {
         const uint32_t tile = first + k;
         if (tile >= it.tiles) break;  // uniform
