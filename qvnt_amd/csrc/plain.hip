@@ -74,6 +74,101 @@ __global__ __launch_bounds__(kMaxBlock) void k_diag(cplx<R>* __restrict__ psi, u
   }
 }
 
+// FP32 diagonals, two amplitudes per 16-byte access: a work item is the
+// group indices 2t and 2t + 1, which the deposit maps to neighbouring
+// amplitudes idx and idx + 1 when bit 0 is not a fixed (control) bit
+// (launch_kernel checks that, and that the group count is even); idx is
+// even, so the access is 16-byte aligned.  One 8-byte amplitude per lane
+// left FP32 diagonals at 5.2 TB/s where FP64 ones (16 bytes per lane)
+// reach 6.4; two per lane: 5.8 TB/s (32 qubits, TFIM ring).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+template <int K, int U, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_diag_x2(cplx<float>* __restrict__ psi, uint64_t pairs, Deposit dep,
+                                                       Par g) {
+  const uint64_t B = blockDim.x, per = B * U;
+  const uint64_t nchunks = (pairs + per - 1) / per;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t base = chunk * per + threadIdx.x;
+    f32x4_t q[U];
+    uint64_t idx[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * B;
+        idx[u] = deposit(2 * t, dep);
+        in[u] = t < pairs;
+        const f32x4_t* a = reinterpret_cast<const f32x4_t*>(psi + idx[u]);
+        if (in[u]) q[u] = NT ? __builtin_nontemporal_load(a) : *a;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!in[u]) continue;
+        cplx<float> a{q[u].x, q[u].y}, b{q[u].z, q[u].w};
+        if (acts(g, idx[u])) a = f_diag<K>(g, idx[u], a);
+        if (acts(g, idx[u] + 1)) b = f_diag<K>(g, idx[u] + 1, b);
+        const f32x4_t o = {a.re, a.im, b.re, b.im};
+        f32x4_t* d = reinterpret_cast<f32x4_t*>(psi + idx[u]);
+        if (NT) __builtin_nontemporal_store(o, d);
+        else *d = o;
+    }
+  }
+}
+
+// FP32 pairs, two orbits per work item (2t and 2t + 1: neighbouring
+// amplitudes i0, i0 + 1 and their partners, when bit 0 is neither fixed
+// nor in the pair mask; launch_kernel checks that), so every access is 16
+// bytes, as in k_diag_x2.
+template <int K, int U, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_pair_x2(cplx<float>* __restrict__ psi, uint64_t pairs, Deposit dep,
+                                                       Par g) {
+  const uint64_t B = blockDim.x, per = B * U;
+  const uint64_t nchunks = (pairs + per - 1) / per;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t base = chunk * per + threadIdx.x;
+    f32x4_t q0[U], q1[U];
+    uint64_t i0[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t t = base + (uint64_t)u * B;
+        i0[u] = deposit(2 * t, dep);
+        in[u] = t < pairs;
+        if (in[u]) {
+            const f32x4_t* a0 = reinterpret_cast<const f32x4_t*>(psi + i0[u]);
+            const f32x4_t* a1 = reinterpret_cast<const f32x4_t*>(psi + (i0[u] ^ g.a));
+            q0[u] = NT ? __builtin_nontemporal_load(a0) : *a0;
+            q1[u] = NT ? __builtin_nontemporal_load(a1) : *a1;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!in[u]) continue;
+        const uint64_t j0 = i0[u], j1 = i0[u] ^ g.a;
+        cplx<float> x0{q0[u].x, q0[u].y}, y0{q0[u].z, q0[u].w};   // j0, j0 + 1
+        cplx<float> x1{q1[u].x, q1[u].y}, y1{q1[u].z, q1[u].w};   // j1, j1 + 1
+        cplx<float> ox0 = x0, ox1 = x1, oy0 = y0, oy1 = y1;
+        if (acts(g, j0)) {
+            ox0 = f_pair<K>(g, j0, x0, x1);
+            ox1 = f_pair<K>(g, j1, x1, x0);
+        }
+        if (acts(g, j0 + 1)) {
+            oy0 = f_pair<K>(g, j0 + 1, y0, y1);
+            oy1 = f_pair<K>(g, j1 + 1, y1, y0);
+        }
+        const f32x4_t o0 = {ox0.re, ox0.im, oy0.re, oy0.im}, o1 = {ox1.re, ox1.im, oy1.re, oy1.im};
+        f32x4_t* d0 = reinterpret_cast<f32x4_t*>(psi + j0);
+        f32x4_t* d1 = reinterpret_cast<f32x4_t*>(psi + j1);
+        if (NT) {
+            __builtin_nontemporal_store(o0, d0);
+            __builtin_nontemporal_store(o1, d1);
+        } else {
+            *d0 = o0;
+            *d1 = o1;
+        }
+    }
+  }
+}
+
 template <int K, class R, int U, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_pair(cplx<R>* __restrict__ psi, uint64_t groups, Deposit dep, Par g) {
   const uint64_t B = blockDim.x, per = B * U;
@@ -395,6 +490,63 @@ int launch_kernel(qvnt_reg* r, const qvnt_gate& gate, GClass cls, uint64_t group
     case KIND:                                                             \
         hipLaunchKernelGGL((KERNEL<KIND, R, U, NT>), grid, block, 0, s, psi, groups, d, p); \
         break;
+    if constexpr (sizeof(R) == 4) {
+        if (cls == GClass::Diag && (d.zeros & 1u) == 0 && (groups & 1u) == 0) {
+            const uint64_t pairs = groups / 2;
+            const uint64_t nx2 = (pairs + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+            const dim3 gx2((unsigned)(nx2 < kMaxGrid ? nx2 : kMaxGrid));
+            cplx<float>* pf = reinterpret_cast<cplx<float>*>(psi);
+#define QVNT_X2(KIND)                                                                        \
+    case KIND:                                                                               \
+        hipLaunchKernelGGL((k_diag_x2<KIND, U, NT>), gx2, block, 0, s, pf, pairs, d0, p);    \
+        break;
+            switch (gate.kind) {
+                QVNT_X2(QVNT_Z)
+                QVNT_X2(QVNT_S)
+                QVNT_X2(QVNT_T)
+                QVNT_X2(QVNT_RZ)
+                QVNT_X2(QVNT_RZZ)
+                QVNT_X2(QVNT_PHI)
+                default: return fail(QVNT_E_ARG, "not a diagonal kind");
+            }
+#undef QVNT_X2
+            QVNT_HIP(hipGetLastError());
+            return QVNT_OK;
+        }
+        // (pairs only for states above kSkewMaxBytes: below, the skewed
+        // one-amplitude kernel is faster -- TFIM ring FP32 at 32 qubits
+        // (32 GiB): 86.3 gates/s against 83.8; at 35 qubits the two-amplitude
+        // kernel wins, 11.35 against 11.07)
+        if (cls == GClass::Pair && r->len * sizeof(cplx<R>) > kSkewMaxBytes && (d.zeros & 1u) == 0 && (p.a & 1u) == 0 &&
+            (groups & 1u) == 0) {
+            const uint64_t pairs = groups / 2;
+            const uint64_t nx2 = (pairs + (uint64_t)B * U - 1) / ((uint64_t)B * U);
+            const dim3 gx2((unsigned)(nx2 < kMaxGrid ? nx2 : kMaxGrid));
+            cplx<float>* pf = reinterpret_cast<cplx<float>*>(psi);
+#define QVNT_X2(KIND)                                                                        \
+    case KIND:                                                                               \
+        hipLaunchKernelGGL((k_pair_x2<KIND, U, NT>), gx2, block, 0, s, pf, pairs, d0, p);    \
+        break;
+            switch (gate.kind) {
+                QVNT_X2(QVNT_X)
+                QVNT_X2(QVNT_Y)
+                QVNT_X2(QVNT_RX)
+                QVNT_X2(QVNT_RXX)
+                QVNT_X2(QVNT_RY)
+                QVNT_X2(QVNT_RYY)
+                QVNT_X2(QVNT_H1)
+                QVNT_X2(QVNT_U1)
+                QVNT_X2(QVNT_SWAP)
+                QVNT_X2(QVNT_ISWAP)
+                QVNT_X2(QVNT_SQRT_SWAP)
+                QVNT_X2(QVNT_SQRT_ISWAP)
+                default: return fail(QVNT_E_ARG, "not a pair kind");
+            }
+#undef QVNT_X2
+            QVNT_HIP(hipGetLastError());
+            return QVNT_OK;
+        }
+    }
     if (cls == GClass::Diag) {
         switch (gate.kind) {
             QVNT_CASE(QVNT_Z, k_diag)

@@ -295,6 +295,12 @@ def test_largest_register(n, prec):
         r.apply(op.x(top))                       # plain: flip the top qubit
         r.apply(op.h(1) * op.h(1))               # h h = identity (exact to rounding)
         r.apply(op.x(2).c(top))                  # control set: fires
+        # pair kernels on high qubits (FP32 past 32 GiB: two amplitudes per
+        # access), each followed by its inverse: identity to rounding
+        hi2, hi3 = top >> 1, top >> 2
+        r.apply(op.ry(0.9, hi2) * op.ry(-0.9, hi2))
+        r.apply(op.rxx(0.5, top | hi3) * op.rxx(-0.5, top | hi3))
+        r.apply(op.swap(hi2 | hi3) * op.swap(hi2 | hi3))
         r.apply_fused(op.x(1).c(top) * op.rz(0.7, top) * op.ry(0.4, 1 << 5) * op.ry(-0.4, 1 << 5))
         want = start ^ top ^ 2 ^ 1
         assert abs(r.get_absolute() - 1.0) < (1e-12 if prec == "f64" else 1e-5)
