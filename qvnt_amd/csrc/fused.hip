@@ -222,12 +222,11 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
     // the product of the phase's G factors (uniform, accumulated in FP64)
     cplx<double> scal{1.0, 0.0};
     bool scaled = false;
-    // the next op's fields are loaded while the current one runs
-    FOp next;
-    if (ph.op_begin < ph.op_end) next = ops[ph.op_begin];
+    // Each op's header is loaded when the op starts.  Loading the next one
+    // ahead held 16 more SGPRs through the op body and doubled the SGPR
+    // spills (into VGPR lanes); the whole-circuit passes ran 3% slower.
     for (uint32_t o = ph.op_begin; o < ph.op_end; ++o) {
-        const FOp op = next;
-        if (o + 1 < ph.op_end) next = ops[o + 1];
+        const FOp op = ops[o];
         // controls on lane bits (this thread's base) and on bits outside the tile
         if ((tb & op.lctrl) != op.lctrl || (tile & op.gctrl) != op.gctrl) continue;
         const double* c = coef + op.coef;
