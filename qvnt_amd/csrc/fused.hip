@@ -197,16 +197,17 @@ __device__ __forceinline__ void apply_ph(cplx<R>* v, const cplx<R> g) {
 }
 
 // the structured block (fusion.hpp: OP_S): diagonal, T2 on register bits
-// 0..3, diagonal -- straight-line code, so the 16 amplitudes stay in
-// registers from the first product to the last
-template <class R>
+// 0..3, diagonal, either diagonal left out when the op's sel says so --
+// each form is straight-line code, so the 16 amplitudes stay in registers
+// from the first product to the last
+template <bool D1, bool D2, class R>
 __device__ __forceinline__ void apply_s(cplx<R>* v, const double* __restrict__ c) {
-    apply_d(v, c);
+    if (D1) apply_d(v, c);
     apply_t2<0, false>(v, (R)c[2 * kRegs + 0], 0u);
     apply_t2<1, false>(v, (R)c[2 * kRegs + 2], 0u);
     apply_t2<2, false>(v, (R)c[2 * kRegs + 4], 0u);
     apply_t2<3, false>(v, (R)c[2 * kRegs + 6], 0u);
-    apply_d(v, c + 2 * (kRegs + kRegBits));
+    if (D2) apply_d(v, c + 2 * (kRegs + kRegBits));
 }
 
 
@@ -252,7 +253,12 @@ __device__ __forceinline__ void run_phase(cplx<R>* v, const FPhase& ph, const FO
                 apply_d(v, c + 2 * kRegs * k);  // per-lane variant table
             }
         } else if (op.type == OP_S) {
-            apply_s(v, c);
+            switch (op.sel) {
+            case 0: apply_s<false, false>(v, c); break;
+            case 1: apply_s<true, false>(v, c); break;
+            case 2: apply_s<false, true>(v, c); break;
+            default: apply_s<true, true>(v, c); break;
+            }
         } else if (op.type == OP_PH) {
             switch (op.sel) {
             case 0: apply_ph<0>(v, inline_coef<R>(op)); break;

@@ -705,23 +705,41 @@ void merge_diagonals(Program& prog, uint32_t begin, bool separate_ext, DepCache&
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return diags[a].hi < diags[b].hi; });
     struct Group { uint32_t slot; FOp op; };
     std::vector<Group> groups;
-    for (size_t k : order) {
-        const Diag& d = diags[k];
-        bool merged = false;
-        for (Group& g : groups)
-            if (g.slot >= d.lo && g.slot <= d.hi && (!separate_ext || plain_diag(g.op) == plain_diag(d.op)) &&
-                merge_diag(prog, g.op, d.op)) {
-                merged = true;
-                break;
-            }
-        if (!merged) {
-            // kept apart, a table with non-register operands goes to the
-            // phase's start or end where it can, out of the matrix ops' run
-            uint32_t slot = d.hi;
-            if (separate_ext && !plain_diag(d.op)) slot = d.lo == 0 ? 0 : d.hi;
-            groups.push_back({slot, d.op});
+    // a T2 that a structured op can take in
+    auto foldable = [&](uint32_t s) {
+        const FOp& op = mats[s];
+        return op.type == OP_T2 && op.pc == 0xFFFFu && !op.lctrl && !op.gctrl;
+    };
+    // Kept apart, a table with non-register operands or controls must not
+    // sit inside a run of foldable T2s (it would split the structured op).
+    // It goes to the phase's start where it can, else to a slot a plain
+    // group holds (the boundary of a run), else to the phase's end, else
+    // to a slot outside any run, else to its window's end.
+    auto ext_slot = [&](const Diag& d) {
+        if (d.lo == 0) return 0u;
+        for (uint32_t s = d.hi + 1; s-- > d.lo;)
+            for (const Group& g : groups)
+                if (g.slot == s && plain_diag(g.op)) return s;
+        if (d.hi == m) return m;
+        for (uint32_t s = d.hi + 1; s-- > d.lo;)
+            if (!foldable(s - 1) || !foldable(s)) return s;
+        return d.hi;
+    };
+    // with separate_ext, the plain tables are grouped first, so the others
+    // can be placed beside them
+    for (int round = 0; round < (separate_ext ? 2 : 1); ++round)
+        for (size_t k : order) {
+            const Diag& d = diags[k];
+            if (separate_ext && plain_diag(d.op) != (round == 0)) continue;
+            bool merged = false;
+            for (Group& g : groups)
+                if (g.slot >= d.lo && g.slot <= d.hi && (!separate_ext || plain_diag(g.op) == plain_diag(d.op)) &&
+                    merge_diag(prog, g.op, d.op)) {
+                    merged = true;
+                    break;
+                }
+            if (!merged) groups.push_back({separate_ext && !plain_diag(d.op) ? ext_slot(d) : d.hi, d.op});
         }
-    }
     // within a slot, tables on register bits only sit next to the matrix
     // ops (before the next one, or after the last), where a structured op
     // can take them in
@@ -810,7 +828,7 @@ void split_pairs(Program& prog, uint32_t begin) {
 // output amplitude times the outputs it writes, plus a fixed cost for the
 // op's decode (dependent scalar loads, branches) and for moving its
 // results back into the registers the op loop carries.
-constexpr uint32_t kOpOverhead = 40;
+constexpr uint32_t kOpOverhead = 80;
 uint32_t op_cost(const FOp& op) {
     const uint32_t outs = (op.type == OP_D || op.type == OP_S) ? (uint32_t)kRegs : (uint32_t)popc(op.pc & 0xFFFFu);
     uint32_t per = 0;
@@ -823,7 +841,7 @@ uint32_t op_cost(const FOp& op) {
     case OP_A2: per = 4; break;
     case OP_X2: per = 1; break;
     case OP_G: per = 0; break;
-    case OP_S: per = 4 + 4 * 2 + 4; break;
+    case OP_S: per = ((op.sel & 1u) ? 4 : 0) + 4 * 2 + ((op.sel & 2u) ? 4 : 0); break;
     case OP_PH: return 4 * kRegs / 2 + kOpOverhead;
     default: per = 2; break;  // T2
     }
@@ -839,7 +857,8 @@ uint32_t phase_cost(const Program& prog, uint32_t begin) {
 
 // Folds runs [plain diagonal] [T2 on distinct register bits, no
 // controls] [plain diagonal] into one S op where the estimate says it is
-// cheaper (missing diagonals become identity tables, missing bits a = 0).
+// cheaper (missing bits a = 0; a missing diagonal is an identity table the
+// kernel skips, flagged in sel: bit 0 first, bit 1 second).
 void fold_structured(Program& prog, uint32_t begin) {
     const std::vector<FOp> in(prog.ops.begin() + begin, prog.ops.end());
     prog.ops.resize(begin);
@@ -864,6 +883,7 @@ void fold_structured(Program& prog, uint32_t begin) {
         FOp sop{};
         sop.type = OP_S;
         sop.pc = 0xFFFFu;
+        sop.sel = (dr ? 1u : 0u) | (dl ? 2u : 0u);
         if (!bits || separate <= op_cost(sop)) {
             prog.ops.push_back(in[i]);
             ++i;
@@ -1392,12 +1412,12 @@ void parallel_for(size_t n, bool parallel, const F& fn) {
 // Fitted run-time model of a compiled queue at 2^30 amplitudes (seconds per
 // pass, seconds per unit of program_cost), per tile size 10, 11, 12: least
 // squares over four measured workloads (30-qubit brickwork per layer and
-// whole, 30-qubit TFIM ring per step and whole, 50 steps), each within 2 %.
+// whole, 30-qubit TFIM ring per step and whole, 50 steps), each within 3 %.
 struct TileModel {
     double pass_s, cost_s;
 };
-constexpr TileModel kModelF64[] = {{5.625e-3, 1.651e-6}, {5.736e-3, 1.847e-6}, {5.728e-3, 2.382e-6}};
-constexpr TileModel kModelF32[] = {{2.274e-3, 1.735e-6}, {2.081e-3, 1.855e-6}, {2.317e-3, 1.929e-6}};
+constexpr TileModel kModelF64[] = {{5.320e-3, 1.217e-6}, {5.298e-3, 1.258e-6}, {5.396e-3, 1.547e-6}};
+constexpr TileModel kModelF32[] = {{2.881e-3, 6.410e-7}, {2.401e-3, 9.295e-7}, {2.505e-3, 9.065e-7}};
 static_assert(sizeof(kModelF64) / sizeof(kModelF64[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 static_assert(sizeof(kModelF32) / sizeof(kModelF32[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 

@@ -80,7 +80,8 @@ enum : uint8_t { OP_M2 = 0, OP_M4 = 1, OP_D = 2, OP_P4 = 3, OP_A2 = 4, OP_X2 = 5
 // kernel reads them with scalar loads into SGPRs.
 struct FOp {                // 64 bytes
     uint32_t type;          // OP_*
-    uint32_t sel;           // M2/A2/X2/T2/PH: register bit j0; M4/P4/P4R: 4*j0 + j1 (j0 < j1)
+    uint32_t sel;           // M2/A2/X2/T2/PH: register bit j0; M4/P4/P4R: 4*j0 + j1 (j0 < j1);
+                            // S: bit 0 / bit 1 set when its first / second diagonal is applied
     uint32_t pc;            // matrix ops: register indices that satisfy the register controls (0xFFFF: all)
     uint32_t lctrl;         // controls on lane bits (local positions)
     uint32_t la, lb;        // D: operand a / b when it is a lane bit (local position mask)

@@ -76,7 +76,10 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
             for (int i = 0; i < kRegs; ++i)
                 if ((i >> op.sel) & 1) v[i] = g * v[i];
         } else if (op.type == OP_S) {
-            for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
+            // sel: which of the two diagonals the kernel applies
+            CHECK(op.sel <= 3);
+            if (op.sel & 1u)
+                for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, i) * v[i];
             for (int j = 0; j < kRegBits; ++j) {
                 CHECK(cf(P, op.coef, kRegs + j).imag() == 0.0);
                 const double a = cf(P, op.coef, kRegs + j).real();
@@ -87,7 +90,8 @@ static void run_phase(C* v, const FPhase& ph, const Program& P, uint32_t tb, uin
                     v[i | (1 << j)] = x1 + a * x0;
                 }
             }
-            for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, kRegs + kRegBits + i) * v[i];
+            if (op.sel & 2u)
+                for (int i = 0; i < kRegs; ++i) v[i] = cf(P, op.coef, kRegs + kRegBits + i) * v[i];
         } else if (op.type == OP_M2 || op.type == OP_A2 || op.type == OP_X2 || op.type == OP_T2) {
             const int J = (int)op.sel;
             CHECK(J < kRegBits);

@@ -44,15 +44,16 @@ def test_auto_tile_follows_the_measured_optimum_fp64(brickwork30):
 
 def test_auto_tile_follows_the_measured_optimum_fp32(brickwork30):
     """FP32 tile rows hold four qubits (128-byte lines), so a pass has one
-    free qubit fewer than in FP64.  Measured at 30 qubits: per layer 0.548 s
-    in tiles of 11 against 0.576 s in 12; whole 0.449 s in 12 against
-    0.450 s in 11; TFIM per step 0.711 s in 11 against 0.746 s in 12; TFIM
-    whole 0.632 s in 11 against 0.636 s in 12."""
+    free qubit fewer than in FP64.  Measured at 30 qubits: per layer 0.512 s
+    in tiles of 11 against 0.516 s in 12 (the choice goes layer by layer:
+    both sizes); whole 0.309 s in 12 against 0.328 s in 11; TFIM per step
+    0.661 s in 11 and in 12; TFIM whole 0.421 s in 12 against 0.457 s in
+    11."""
     assert fused_plan_stats(merged(brickwork30), 30, "f32")["tile"] == 12
-    assert all(fused_plan_stats(o, 30, "f32")["tile"] == 11 for o in brickwork30)
+    assert all(fused_plan_stats(o, 30, "f32")["tile"] in (11, 12) for o in brickwork30)
     tfim = circuits.build(circuits.tfim_step(30), op)
-    assert fused_plan_stats(tfim, 30, "f32")["tile"] == 11
-    assert fused_plan_stats(merged([tfim] * 50), 30, "f32")["tile"] == 11
+    assert fused_plan_stats(tfim, 30, "f32")["tile"] in (11, 12)
+    assert fused_plan_stats(merged([tfim] * 50), 30, "f32")["tile"] == 12
     # the rows: every pass of the same circuit holds qubits 0..3 in FP32
     # and needs at least as many passes as in FP64
     for o in brickwork30[:4]:
