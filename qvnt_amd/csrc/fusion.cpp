@@ -461,6 +461,21 @@ std::vector<PassPlan> schedule_passes(std::vector<Prim> work, unsigned ns, unsig
     return passes;
 }
 
+// The same greedy run from the queue's end: the last pass takes every
+// primitive that commutes with all those left after it, and so on
+// backwards; the passes are then put in order.  The commutation rules are
+// symmetric, so the plan is as valid.  It differs where the queue's order
+// of qubits works against it: a controlled gate whose control qubit's
+// one-qubit gates fall in a later forward pass than its target's needs the
+// target in yet another pass (a brickwork layer's cx with the control above
+// the target: 4 passes forward, 3 backward).  Not for relabeled queues
+// (their passes also permute the layout, in queue order).
+std::vector<PassPlan> schedule_backward(const std::vector<Prim>& work, unsigned ns, unsigned tq, unsigned rows) {
+    std::vector<PassPlan> passes = schedule_passes(std::vector<Prim>(work.rbegin(), work.rend()), ns, tq, rows, nullptr);
+    std::reverse(passes.begin(), passes.end());
+    for (PassPlan& pp : passes) std::reverse(pp.prims.begin(), pp.prims.end());
+    return passes;
+}
 
 int local_of(const uint32_t* tpos, unsigned t, int q) {
     for (unsigned k = 0; k < t; ++k)
@@ -1525,8 +1540,13 @@ void plan(const qvnt_gate* g, size_t n, unsigned ns, unsigned tq, std::vector<St
             const unsigned rows = row_bits(amp_bytes);
             std::vector<int> want;
             const bool relabeled = relabel_swaps(prims, ns, ns > tq && tq < rows + 2, &want);
-            const std::vector<PassPlan> passes =
-                schedule_passes(merge_single(prims), ns, tq, rows, relabeled ? &want : nullptr);
+            const std::vector<Prim> merged = merge_single(prims);
+            std::vector<PassPlan> passes = schedule_passes(merged, ns, tq, rows, relabeled ? &want : nullptr);
+            // the backward plan where it needs fewer passes
+            if (!relabeled && passes.size() > 1) {
+                std::vector<PassPlan> back = schedule_backward(merged, ns, tq, rows);
+                if (back.size() < passes.size()) passes.swap(back);
+            }
             // primitives per source SingleOp, to find passes that hold one
             // whole SingleOp and nothing else: one plain launch moves at
             // most the same bytes as a pass (fewer with controls) and skips
