@@ -295,16 +295,21 @@ int build_schedule(qvnt_batch* b, const std::vector<RegPlan>& plans, const std::
     uint32_t widest = 0;
     for (uint32_t r = 0; r < S.rounds; ++r) widest = std::max(widest, p_rounds[r + 1] - p_rounds[r]);
     uint32_t resident = 0;
-    if (S.tmax >= (uint32_t)qvnt::fusion::kMinFused && qvnt::batch_persist_grid(b->prec, S.tmax, &resident) == QVNT_OK)
+    if (S.tmax >= (uint32_t)qvnt::fusion::kMinFused && qvnt::batch_persist_grid(b->prec, S.tmax, &resident) == QVNT_OK) {
         S.persist_blocks = std::min(resident, widest);
-    else
+        // a multiple of 8: every XCD gets as many blocks (k_batch_persist
+        // hands each XCD its own range of a round's units)
+        if (S.persist_blocks >= 8) S.persist_blocks &= ~7u;
+    } else {
         qvnt::clear_error();   // the persistent form is optional: the rounds run without it
+    }
     S.p_items_off = 0;
     S.p_units_off = a256(p_items.size() * sizeof(BItem));
     S.p_rounds_off = a256(S.p_units_off + p_units.size() * sizeof(PUnit));
     S.p_status_off = a256(S.p_rounds_off + p_rounds.size() * sizeof(uint32_t));
-    // (the status word starts at zero: phost is zeroed)
-    std::vector<unsigned char> phost(a256(S.p_status_off + sizeof(uint32_t)));
+    // (the status word and the rounds' claim counters start at zero: phost
+    // is zeroed)
+    std::vector<unsigned char> phost(a256(S.p_status_off + (1 + S.rounds) * sizeof(uint32_t)));
     if (!p_items.empty()) std::memcpy(phost.data() + S.p_items_off, p_items.data(), p_items.size() * sizeof(BItem));
     if (!p_units.empty()) std::memcpy(phost.data() + S.p_units_off, p_units.data(), p_units.size() * sizeof(PUnit));
     std::memcpy(phost.data() + S.p_rounds_off, p_rounds.data(), p_rounds.size() * sizeof(uint32_t));

@@ -647,18 +647,60 @@ __global__ QVNT_FUSED_BOUNDS void k_batch(cplx<R>* __restrict__ arena, const uns
     }
 }
 
+// One work unit of the persistent batched kernel (below): up to
+// 2^(tmax-t) tiles of one register pass of tile size t, one tile per group
+// of 2^(t-4) threads with its own 2^t-amplitude LDS region; all groups run
+// the same program, so they pass the same barriers, and a group without a
+// tile runs the phases without HBM traffic.  Groups of whole waves (t >= 10)
+// take their tile index wave-uniform, as in k_batch, so the program's
+// tile-dependent branches and table loads stay scalar.
+template <class R>
+__device__ __forceinline__ void persist_unit(cplx<R>* __restrict__ arena, const unsigned char* __restrict__ prog,
+                                             const BItem* __restrict__ items, const PUnit& un, unsigned char* smem) {
+    const BItem it = items[un.item];
+    const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
+    const FPhase* ph = reinterpret_cast<const FPhase*>(prog + it.phase_off);
+    const FOp* ops = reinterpret_cast<const FOp*>(prog + it.op_off);
+    const double* coef = reinterpret_cast<const double*>(prog + it.coef_off);
+    const uint32_t tbits = pass.t - kRegBits;                 // threads per tile: 2^tbits
+    const uint32_t tid = threadIdx.x & ((1u << tbits) - 1u);
+    if (tbits >= 6) {
+        const uint32_t group = __builtin_amdgcn_readfirstlane(threadIdx.x >> tbits);
+        const uint32_t tile = un.tile0 + group;
+        fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                                   reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+    } else {
+        const uint32_t group = threadIdx.x >> tbits;
+        const uint32_t tile = un.tile0 + group;
+        fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
+                                   reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+    }
+}
+
+// Units of each round (in thousandths) that blocks claim one at a time
+// instead of taking them in a fixed order (4096-register batch, FP64:
+// 54.2 ms per run with none, 51.6 with 100, 50.7 with 150, 53.1 with 250).
+constexpr uint32_t kPersistClaimed = 150;
+
 // The batched form as one persistent launch (cooperative: every block is
 // resident).  The blocks walk the rounds in order and meet at a grid
 // barrier between rounds, which also makes a round's HBM writes visible to
 // the next (cooperative groups' grid sync: release and acquire fences at
-// device scope).  Block b runs units b, b + gridDim.x, ... of each round.
-// A block of 2^(tmax-4) threads runs one unit: up to 2^(tmax-t) tiles of
-// one register pass of tile size t, one tile per group of 2^(t-4) threads
-// with its own 2^t-amplitude LDS region; all groups run the same program,
-// so they pass the same barriers, and a group without a tile runs the
-// phases without HBM traffic.  Groups of whole waves (t >= 10) take their
-// tile index wave-uniform, as in k_batch, so the program's tile-dependent
-// branches and table loads stay scalar.
+// device scope).  In each round, a whole number of strides of the round's
+// first units go out in a fixed order: with a grid of 8q blocks, XCD x (the
+// blocks b with b mod 8 = x, which the dispatcher places on XCD x) walks
+// its own contiguous eighth of them, its q blocks q apart.  The rest (about
+// kPersistClaimed thousandths) are claimed one at a time from the round's
+// counter, so blocks that finish early take more of the round's tail.
+// status[1 + r] is round r's counter: zero when the launch starts, zeroed
+// for round r + 1 by block 0 during round r (nobody claims from it before
+// the barrier that ends round r), and round 0's zeroed again by block 0 at
+// the end, after every block has left it; so the counters are zero again
+// when the launch ends.
+//
+// The dynamic LDS array starts 16-byte aligned after the claim word: the
+// tile's 16-byte accesses need it (an array at byte 4 made every run 35 %
+// slower).
 //
 // The launch bounds are k_fused's: they leave the register allocation at
 // what the kernel uses, which is what the runtime's occupancy query (and so
@@ -673,40 +715,50 @@ __global__ QVNT_FUSED_BOUNDS void k_batch_persist(cplx<R>* __restrict__ arena, c
                                                   const BItem* __restrict__ items, const PUnit* __restrict__ units,
                                                   const uint32_t* __restrict__ round_begin, uint32_t rounds,
                                                   uint32_t* status) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t claim;
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     if (!grid.is_valid()) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *status = 1u;
         return;
     }
+    uint32_t* count = status + 1;
+    const uint32_t G = gridDim.x;
+    const bool by_xcd = (G & 7u) == 0;
+    const uint32_t xcd = blockIdx.x & 7u;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t u_end = round_begin[r + 1];
-        for (uint32_t u = round_begin[r] + blockIdx.x; u < u_end; u += gridDim.x) {
-            const PUnit un = units[u];
-            const BItem it = items[un.item];
-            const FPass& pass = *reinterpret_cast<const FPass*>(prog + it.pass_off);
-            const FPhase* ph = reinterpret_cast<const FPhase*>(prog + it.phase_off);
-            const FOp* ops = reinterpret_cast<const FOp*>(prog + it.op_off);
-            const double* coef = reinterpret_cast<const double*>(prog + it.coef_off);
-            const uint32_t tbits = pass.t - kRegBits;                 // threads per tile: 2^tbits
-            const uint32_t tid = threadIdx.x & ((1u << tbits) - 1u);
-            if (tbits >= 6) {
-                const uint32_t group = __builtin_amdgcn_readfirstlane(threadIdx.x >> tbits);
-                const uint32_t tile = un.tile0 + group;
-                fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
-                                    reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+        const uint32_t rb = round_begin[r], u_end = round_begin[r + 1], n = u_end - rb;
+        if (blockIdx.x == 0 && threadIdx.x == 0 && r + 1 < rounds) count[r + 1] = 0u;
+        const uint32_t claimed = (uint32_t)(((uint64_t)n * kPersistClaimed) / 1000u);
+        const uint32_t fixed = ((n - claimed) / G) * G;       // whole strides
+        // this block's units of the fixed part: [next, stop) in steps of step
+        uint32_t next = rb + blockIdx.x, stop = rb + fixed, step = G;
+        if (by_xcd) {
+            next = rb + xcd * (fixed >> 3) + (blockIdx.x >> 3);
+            stop = rb + (xcd + 1u) * (fixed >> 3);
+            step = G >> 3;
+        }
+        for (;;) {
+            uint32_t u;
+            if (next < stop) {  // uniform
+                u = next;
+                next += step;
             } else {
-                const uint32_t group = threadIdx.x >> tbits;
-                const uint32_t tile = un.tile0 + group;
-                fused_tile<R, false, true>(arena + it.psi_off, pass, ph, ops, coef, tile,
-                                    reinterpret_cast<cplx<R>*>(smem) + ((size_t)group << pass.t), tid, tile < un.tile_end);
+                if (threadIdx.x == 0) claim = atomicAdd(&count[r], 1u);
+                __syncthreads();
+                const uint32_t k = claim;
+                __syncthreads();  // before the next claim overwrites it
+                if (k >= n - fixed) break;
+                u = rb + fixed + k;
             }
+            persist_unit<R>(arena, prog, items, units[u], smem);
             // the next unit's first LDS writes may hit elements other
             // threads still read in this unit's last phase
             __syncthreads();
         }
         grid.sync();
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[0] = 0u;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }

@@ -108,9 +108,10 @@ namespace fusion { struct PUnit; }
 // 2^tmax, and its cooperative launch over `rounds` rounds of work units
 // (units [round_begin[r], round_begin[r+1]) form round r; device pointers)
 int batch_persist_grid(qvnt_prec prec, uint32_t tmax, uint32_t* blocks);
-// (status: a device word the kernel sets to 1 if its launch was not
-// cooperative; graph non-null: capture the launch into *graph instead of
-// running it)
+// (status: 1 + rounds device words, zero at the first launch: the kernel
+// sets status[0] to 1 if its launch was not cooperative, and uses
+// status[1 + r] as round r's claim counter, leaving it zero; graph
+// non-null: capture the launch into *graph instead of running it)
 int launch_batch_persist(hipStream_t stream, qvnt_prec prec, void* arena, const void* prog, const fusion::BItem* items,
                          const fusion::PUnit* units, const uint32_t* round_begin, uint32_t rounds, uint32_t tmax,
                          uint32_t blocks, uint32_t* status, hipGraph_t* graph);

@@ -131,6 +131,31 @@ def test_persistent_launch_equals_rounds(prec, cap, dense):
                 assert np.array_equal(b.get_state(k), want[k]), "run %d, register %d (%d qubits)" % (rep, k, n)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_persistent_fixed_and_claimed_units(prec):
+    """Rounds with more units than the grid has blocks: each round's units
+    split into a fixed part (each XCD's blocks walk their own range) and a
+    tail that blocks claim one at a time from the round's counter.  The
+    launch equals the round launches bit for bit, three launches in a row
+    (the counters must return to zero after each)."""
+    regs, states, ops = workload(128, 77, n_lo=15, n_hi=17, g_lo=20, g_hi=80)
+    with Batch([n for n, _ in regs], states, prec=prec) as b:
+        info = b.compile(ops, max_qubits=10, dense_min=0)
+        G = info["persist_blocks"]
+        assert G > 0 and G % 8 == 0
+        # round 0 holds one unit per tile of every register's first pass
+        assert sum(1 << (n - 10) for n, _ in regs) * 0.85 >= 2 * G
+        b.run()
+        want = [b.get_state(k) for k in range(len(regs))]
+        for rep in range(3):
+            b.reset()
+            b.run(graph="persistent")
+            b.sync()
+            assert b.persist_form == "graph"
+            for k, (n, _) in enumerate(regs):
+                assert np.array_equal(b.get_state(k), want[k]), "launch %d, register %d (%d qubits)" % (rep, k, n)
+
+
 @pytest.mark.parametrize("dense,cap", [(0, 12), (2, 12), (0, 13), (3, 13)])
 def test_fp32_batch_error_bound(dense, cap):
     regs, states, ops = workload(24, 9 + dense + cap, n_lo=8, n_hi=16)
