@@ -677,10 +677,13 @@ __device__ __forceinline__ void persist_unit(cplx<R>* __restrict__ arena, const 
     }
 }
 
-// Units of each round (in thousandths) that blocks claim one at a time
-// instead of taking them in a fixed order (4096-register batch, FP64:
-// 54.2 ms per run with none, 51.6 with 100, 50.7 with 150, 53.1 with 250).
+// Units of each round (in thousandths) that blocks claim instead of taking
+// them in a fixed order, kPersistChunk consecutive units per claim
+// (4096-register batch, FP64: 54.2 ms per run with none; one unit per
+// claim: 51.6 with 100, 50.7 with 150, 53.1 with 250; with 150, two units
+// per claim 48.3, four 48.3; 300 in fours 48.3, 400 in fours 48.5).
 constexpr uint32_t kPersistClaimed = 150;
+constexpr uint32_t kPersistChunk = 4;
 
 // The batched form as one persistent launch (cooperative: every block is
 // resident).  The blocks walk the rounds in order and meet at a grid
@@ -690,8 +693,9 @@ constexpr uint32_t kPersistClaimed = 150;
 // first units go out in a fixed order: with a grid of 8q blocks, XCD x (the
 // blocks b with b mod 8 = x, which the dispatcher places on XCD x) walks
 // its own contiguous eighth of them, its q blocks q apart.  The rest (about
-// kPersistClaimed thousandths) are claimed one at a time from the round's
-// counter, so blocks that finish early take more of the round's tail.
+// kPersistClaimed thousandths) are claimed kPersistChunk consecutive units
+// at a time from the round's counter, so blocks that finish early take
+// more of the round's tail.
 // status[1 + r] is round r's counter: zero when the launch starts, zeroed
 // for round r + 1 by block 0 during round r (nobody claims from it before
 // the barrier that ends round r), and round 0's zeroed again by block 0 at
@@ -738,18 +742,24 @@ __global__ QVNT_FUSED_BOUNDS void k_batch_persist(cplx<R>* __restrict__ arena, c
             stop = rb + (xcd + 1u) * (fixed >> 3);
             step = G >> 3;
         }
+        // the claimed units in hand: [have, have_end)
+        uint32_t have = 0, have_end = 0;
         for (;;) {
             uint32_t u;
             if (next < stop) {  // uniform
                 u = next;
                 next += step;
             } else {
-                if (threadIdx.x == 0) claim = atomicAdd(&count[r], 1u);
-                __syncthreads();
-                const uint32_t k = claim;
-                __syncthreads();  // before the next claim overwrites it
-                if (k >= n - fixed) break;
-                u = rb + fixed + k;
+                if (have >= have_end) {
+                    if (threadIdx.x == 0) claim = atomicAdd(&count[r], 1u);
+                    __syncthreads();
+                    const uint64_t first = (uint64_t)claim * kPersistChunk;
+                    __syncthreads();  // before the next claim overwrites it
+                    if (first >= n - fixed) break;
+                    have = rb + fixed + (uint32_t)first;
+                    have_end = rb + fixed + (uint32_t)min((uint64_t)(n - fixed), first + kPersistChunk);
+                }
+                u = have++;
             }
             persist_unit<R>(arena, prog, items, units[u], smem);
             // the next unit's first LDS writes may hit elements other
