@@ -1425,14 +1425,16 @@ void parallel_for(size_t n, bool parallel, const F& fn) {
 }
 
 // Fitted run-time model of a compiled queue at 2^30 amplitudes (seconds per
-// pass, seconds per unit of program_cost), per tile size 10, 11, 12: least
-// squares over four measured workloads (30-qubit brickwork per layer and
-// whole, 30-qubit TFIM ring per step and whole, 50 steps), each within 3 %.
+// pass, seconds per unit of program_cost), per tile size 10, 11, 12, 13:
+// least squares over four measured workloads (30-qubit brickwork per layer
+// and whole, 30-qubit TFIM ring per step and whole, 50 steps), each within
+// 3 %.  FP64 has no tile of 13 (its entry is never read).
 struct TileModel {
     double pass_s, cost_s;
 };
-constexpr TileModel kModelF64[] = {{5.320e-3, 1.217e-6}, {5.298e-3, 1.258e-6}, {5.396e-3, 1.547e-6}};
-constexpr TileModel kModelF32[] = {{2.881e-3, 6.410e-7}, {2.401e-3, 9.295e-7}, {2.505e-3, 9.065e-7}};
+constexpr TileModel kModelF64[] = {{5.320e-3, 1.217e-6}, {5.298e-3, 1.258e-6}, {5.396e-3, 1.547e-6}, {0.0, 0.0}};
+constexpr TileModel kModelF32[] = {{2.881e-3, 6.410e-7}, {2.401e-3, 9.295e-7}, {2.505e-3, 9.065e-7},
+                                   {3.068e-3, 8.457e-7}};
 static_assert(sizeof(kModelF64) / sizeof(kModelF64[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 static_assert(sizeof(kModelF32) / sizeof(kModelF32[0]) == sizeof(kAutoTiles) / sizeof(kAutoTiles[0]), "model");
 
@@ -1452,7 +1454,7 @@ unsigned plan_auto(const qvnt_gate* g, size_t n, unsigned ns, bool fp64, std::ve
     // the candidates compile side by side
     parallel_for(kTiles, true, [&](size_t k) {
         est[k] = -1.0;
-        if (kAutoTiles[k] > ns) return;
+        if (kAutoTiles[k] > ns || kAutoTiles[k] > max_tile(fp64)) return;
         plan(g, n, ns, kAutoTiles[k], cand[k], 0, single_plain, parallel, fp64 ? 16 : 8);
         const TileModel& m = (fp64 ? kModelF64 : kModelF32)[k];
         double passes = 0.0, cost = 0.0;

@@ -204,8 +204,9 @@ uint64_t program_cost(const Program& prog);
 // counted as a pass), with a_t, b_t per tile size and precision fitted to
 // measured fused applies on MI355X.  Larger tiles need fewer passes;
 // smaller tiles run each op faster (more, smaller blocks per CU hide the
-// barriers between phases).  Returns the tile size kept.
-constexpr unsigned kAutoTiles[] = {10, 11, 12};
+// barriers between phases).  Returns the tile size kept.  (13 only in
+// FP32: an FP64 tile stops at 12, max_tile.)
+constexpr unsigned kAutoTiles[] = {10, 11, 12, 13};
 // From 2^28 amplitudes a pass takes milliseconds, more than compiling the
 // candidates costs; single-register applies choose the tile from there.
 constexpr unsigned kAutoTileQubits = 28;
