@@ -707,13 +707,35 @@ void merge_diagonals(Program& prog, uint32_t begin, bool separate_ext, DepCache&
         else mats.push_back(op);
     }
     const uint32_t m = (uint32_t)mats.size();
-    for (Diag& d : diags) {
-        d.lo = 0;
-        d.hi = m;
-        for (uint32_t k = 0; k < d.at; ++k)
-            if (reg_targets(mats[k]) & d.dep) d.lo = k + 1;
-        for (uint32_t k = m; k-- > d.at;)
-            if (reg_targets(mats[k]) & d.dep) d.hi = k;
+    // windows: lo = 1 + the last matrix op before the diagonal that targets
+    // one of its bits (0 if none), hi = the first one from its place on (m
+    // if none); one sweep each way, per register bit (diags are in op order,
+    // so their `at` never decreases)
+    {
+        uint32_t last[kRegBits] = {0, 0, 0, 0};  // 1 + the last matrix op on bit j so far
+        size_t di = 0;
+        for (uint32_t k = 0; k <= m; ++k) {
+            for (; di < diags.size() && diags[di].at == k; ++di) {
+                Diag& d = diags[di];
+                d.lo = 0;
+                for (int j = 0; j < kRegBits; ++j)
+                    if ((d.dep >> j) & 1u) d.lo = std::max(d.lo, last[j]);
+            }
+            if (k < m)
+                for (uint32_t t = reg_targets(mats[k]); t; t &= t - 1) last[ctz(t)] = k + 1;
+        }
+        uint32_t next[kRegBits] = {m, m, m, m};  // the first matrix op on bit j at or after k
+        di = diags.size();
+        for (uint32_t k = m + 1; k-- > 0;) {
+            for (; di > 0 && diags[di - 1].at == k; --di) {
+                Diag& d = diags[di - 1];
+                d.hi = m;
+                for (int j = 0; j < kRegBits; ++j)
+                    if ((d.dep >> j) & 1u) d.hi = std::min(d.hi, next[j]);
+            }
+            if (k > 0)
+                for (uint32_t t = reg_targets(mats[k - 1]); t; t &= t - 1) next[ctz(t)] = k - 1;
+        }
     }
     std::vector<size_t> order(diags.size());
     for (size_t k = 0; k < order.size(); ++k) order[k] = k;
@@ -1183,6 +1205,7 @@ void compile_pass(Program& prog, const PassPlan& pp_in, unsigned dense_min) {
         uint32_t Rl = 0;  // local register bits
         Left left;
         PhasePlan plan;
+        plan.prims.reserve(rest.size());
         leave.clear();
         size_t i = 0;
         for (; i < rest.size() && (left.mat_targets & support) != support; ++i) {
