@@ -324,6 +324,29 @@ static std::vector<qvnt_gate> layered_circuit(std::mt19937_64& g, unsigned n, in
     return out;
 }
 
+// One brickwork layer on 19 qubits whose cx runs from qubit 18 to qubit 3.
+// In tiles of 11 (rows 0..2 and 8 free qubits a pass) its 16 other qubits
+// fill exactly two passes; formed from the queue's start they put qubit
+// 18's u3 in the second pass and qubit 3's in the first, and the cx then
+// needs qubit 3 in a third pass.  Formed from the queue's end, two passes.
+static std::vector<qvnt_gate> cx_high_layer(std::mt19937_64& g, unsigned n) {
+    std::vector<qvnt_gate> out;
+    std::uniform_real_distribution<double> ang(0.0, 6.283185307179586);
+    qvnt_gate buf[64];
+    for (unsigned q = 0; q < n; ++q) {
+        const long m = orc_op_u3(ang(g), ang(g), ang(g), 1ull << q, buf);
+        out.insert(out.end(), buf, buf + m);
+    }
+    for (unsigned q = 1; q + 1 < n; q += 2) {
+        const long m = orc_op_rzz(ang(g), (1ull << q) | (1ull << (q + 1)), buf);
+        out.insert(out.end(), buf, buf + m);
+    }
+    long m = orc_op_x(1ull << 3, buf);
+    CHECK(orc_op_c(buf, (size_t)m, 1ull << (n - 1)) == 0);
+    out.insert(out.end(), buf, buf + m);
+    return out;
+}
+
 // Programs equal field by field (coefficients bit for bit).
 static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
     if (a.size() != b.size()) return false;
@@ -356,9 +379,10 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
 }
 
 static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false,
-                       bool autotile = false, unsigned amp_bytes = 16) {
+                       bool autotile = false, unsigned amp_bytes = 16, unsigned want_passes = 0) {
     std::mt19937_64 g(seed);
-    const std::vector<qvnt_gate> gates = layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
+    const std::vector<qvnt_gate> gates = want_passes ? cx_high_layer(g, n)
+                                         : layered ? layered_circuit(g, n, 5) : random_circuit(g, n, 160);
     std::vector<Step> steps;
     if (autotile) {
         // the tile size chosen by the run-time model (tq reports it)
@@ -406,6 +430,10 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
             permuted += fp.permuted;
         }
     }
+    if (want_passes && passes + plain != want_passes) {
+        std::printf("n=%u tq=%u: %u steps, want %u\n", n, tq, passes + plain, want_passes);
+        ++failures;
+    }
     const C* want = reinterpret_cast<const C*>(orc_reg_psi(ref));
     double err = 0.0;
     for (uint64_t i = 0; i < len; ++i) err = std::max(err, std::abs(psi[i] - want[i]));
@@ -425,6 +453,8 @@ int main() {
     for (const auto& c : layered) check_case(c[0], c[1], c[2], seed++, true);
     for (unsigned n : {12u, 14u, 16u}) check_case(n, 0, 0, seed++, n == 16, true);
     for (unsigned n : {9u, 13u, 15u}) check_case(n, 11, n == 13 ? 2 : 0, seed++, false, false, 8);
+    // passes formed from the queue's end where that needs fewer
+    check_case(19, 11, 0, seed++, false, false, 16, 2);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
