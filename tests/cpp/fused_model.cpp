@@ -378,6 +378,9 @@ static bool same_steps(const std::vector<Step>& a, const std::vector<Step>& b) {
     return true;
 }
 
+// structured ops run, by form (sel: which of the two diagonals they apply)
+static unsigned s_forms[4] = {0, 0, 0, 0};
+
 static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, bool layered = false,
                        bool autotile = false, unsigned amp_bytes = 16, unsigned want_passes = 0) {
     std::mt19937_64 g(seed);
@@ -423,7 +426,10 @@ static void check_case(unsigned n, unsigned tq, unsigned dense, uint64_t seed, b
             continue;
         }
         for (const FPhase& ph : st.prog.phases) dense_phases += ph.dense;
-        for (const FOp& op : st.prog.ops) split += op.type == OP_T2 || op.type == OP_S;
+        for (const FOp& op : st.prog.ops) {
+            split += op.type == OP_T2 || op.type == OP_S;
+            if (op.type == OP_S && op.sel < 4) ++s_forms[op.sel];
+        }
         for (const FPass& fp : st.prog.passes) {
             run_pass(psi.data(), len, st.prog, fp);
             ++passes;
@@ -455,6 +461,10 @@ int main() {
     for (unsigned n : {9u, 13u, 15u}) check_case(n, 11, n == 13 ? 2 : 0, seed++, false, false, 8);
     // passes formed from the queue's end where that needs fewer
     check_case(19, 11, 0, seed++, false, false, 16, 2);
+    // every form of the structured op is exercised
+    std::printf("structured ops by form: %u %u %u %u\n", s_forms[0], s_forms[1], s_forms[2], s_forms[3]);
+    for (unsigned c : s_forms)
+        if (!c) ++failures;
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
 }
