@@ -10,7 +10,7 @@
  *   QReg::{new, with_state, apply, get_probabilities, get_absolute,
  *          measure_mask, measure}             src/register/quant.rs:113-507
  * Every entry point below names the reference item it replaces.  A Rust
- * `-sys` crate binds these symbols one-to-one (see INTEGRATION.md).
+ * `-sys` crate binds these symbols one-to-one.
  *
  * Conventions
  *   - Every function returning `int` returns QVNT_OK (0) or a negative
