@@ -28,11 +28,12 @@ its device memory before the next:
   batch      4096 registers of 10-20 qubits, FP64 and FP32, autotuned and
              run as one hipGraph: gates/s, ms per run, HBM and matrix-core
              rates (the plain register-by-register loop beside FP64);
-  served     128 serving requests (12-30 qubits, 100-2000 gates, 1024
-             shots): one-at-a-time plain and fused loops, the
-             continuous-batching server at saturation, and open-loop
-             Poisson arrivals at 50/80/100 % of the serial fused capacity
-             into a one-slot and a four-slot server (p50/p99 latency).
+  served     256 serving requests (12-30 qubits, 100-2000 gates, 1024
+             shots): one-at-a-time fused loop (and plain loop on the first
+             128), the continuous-batching server at saturation, and
+             open-loop Poisson arrivals at 50/80/100 % of the serial fused
+             capacity into a one-slot and a four-slot server (p50/p99
+             latency).
 """
 import argparse
 import json
@@ -374,10 +375,12 @@ def tfim_pair_leg(n, steps, forms=("fused", "plain")):
     return out
 
 
-def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0.8, 1.0)):
+def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0.8, 1.0), plain_requests=128):
     """Serving workload (n log-uniform in [12, n_hi], 100-2000 mixed gates,
     1024 seeded shots of measure_mask on 8 random qubits per request):
-      serial_plain   closed loop, one request at a time, plain form;
+      serial_plain   closed loop, one request at a time, plain form, on the
+                     first plain_requests requests (the plain form takes
+                     ~0.3 s a request);
       serial_fused   closed loop, one request at a time, fused form (its
                      circuits/s is the capacity c of serial service);
       saturation     every request submitted at once into the
@@ -400,9 +403,9 @@ def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0
         min(len(o) for o in ops), max(len(o) for o in ops)))
     arena = int(arena_gib * 2**30)
 
-    def stats(res, label):
+    def stats(res, label, count=None):
         bad = [r["id"] for r in res if r["status"] != 0]
-        if bad or len(res) != len(reqs):
+        if bad or len(res) != (count or len(reqs)):
             raise RuntimeError("%s: %d results, failed requests %s" % (label, len(res), bad[:8]))
         lat = np.array([r["t_done"] - r["t_submit"] for r in res])
         span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
@@ -415,10 +418,10 @@ def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0
                                                                         o["p99_ms"], span))
         return o
 
-    def closed_loop(fused):
+    def closed_loop(fused, count=None):
         res = []
         with Server(arena, slots=1) as srv:
-            for k, (n, _, mask, draws) in enumerate(reqs):
+            for k, (n, _, mask, draws) in enumerate(reqs[:count]):
                 srv.submit(k, n, ops[k], mask, draws, fused=fused, dense_min=dense if fused else 0)
                 res += srv.drain(1)
         return res
@@ -447,8 +450,10 @@ def serve_leg(requests, n_hi, slots, arena_gib, dense, plain=True, loads=(0.5, 0
 
     totals_plain = None
     if plain:
-        res = closed_loop(False)
-        out["serial_plain"] = stats(res, "serial plain")
+        n_plain = min(plain_requests, len(reqs))
+        res = closed_loop(False, n_plain)
+        out["serial_plain"] = stats(res, "serial plain (first %d requests)" % n_plain, n_plain)
+        out["serial_plain"]["requests"] = n_plain
         totals_plain = {r["id"]: r["total"] for r in res}
     res = closed_loop(True)
     out["serial_fused"] = stats(res, "serial fused")
@@ -767,7 +772,7 @@ def main():
                     help="fused forms: fold phases of >= this many ops into dense 16x16 unitaries (0: none; "
                          "default 0, serve 3; batch: used with --tile-cap, else autotuned)")
     ap.add_argument("--check-registers", type=int, default=64, help="batch: registers compared with the plain form")
-    ap.add_argument("--requests", type=int, default=128, help="serve: number of requests")
+    ap.add_argument("--requests", type=int, default=256, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=4,
                     help="serve: concurrent requests on the device (one stream each; the runtime has 4 hardware queues)")
     ap.add_argument("--arena-gib", type=float, default=160.0, help="serve: HBM admission budget")
@@ -879,7 +884,7 @@ def main():
         extra["tfim_pair"] = tfim_pair_leg(28, 50)
         extra["batch"] = {"f64": batch_leg(4096, 20, "f64", 0, 0, 20, 2, 64),
                           "f32": batch_leg(4096, 20, "f32", 0, 0, 20, 2, 64, plain=False)}
-        extra["served"] = serve_leg(128, 30, 4, 160.0, 3)
+        extra["served"] = serve_leg(256, 30, 4, 160.0, 3)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "gates/s", "n_gpus": 1,
