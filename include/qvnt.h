@@ -186,8 +186,8 @@ int qvnt_reg_apply_gates(qvnt_reg* reg, const qvnt_gate* gates, size_t count);
  * absorbed while tiles of 2^t amplitudes (t = max_qubits, at most 12 for
  * FP64 and 13 for FP32) sit in LDS and registers.  max_qubits = 0: t = 12,
  * except from 28 qubits without dense phases, where the queue is compiled
- * for t = 10, 11 and 12 and the program with the lowest estimated run time
- * is kept (a model of passes and arithmetic fitted to measured runs; long
+ * for t = 10, 11 and 12 (and 13 in FP32) and the program with the lowest
+ * estimated run time is kept (a model of passes and arithmetic fitted to measured runs; long
  * compute-heavy queues run faster in smaller tiles).  Gates are reordered only past gates on
  * disjoint qubits.  Same result as qvnt_reg_apply to rounding (FMA and
  * merged 2x2 products: about 1e-15 per gate).  *passes (may be NULL) gets
