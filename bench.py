@@ -21,8 +21,9 @@ loop) on the host on a bounded sample of the same workload, plus the
 Further keys of the default run (skipped with --no-extra), each leg freeing
 its device memory before the next:
   tfim       TFIM ring, 34 qubits FP64 and 35 qubits FP32 (256 GiB each):
-             one plain Trotter step with its dominant kernel's roofline, and
-             50 steps through the fused form as one call;
+             one plain Trotter step with its dominant kernel's roofline,
+             4 steps through the fused form one call per step, and 50 steps
+             as one call;
   tfim_pair  28 qubits, FP32 against FP64 after every one of 50 steps
              (fused and plain): max |dpsi| and infidelity;
   batch      4096 registers of 10-20 qubits, FP64 and FP32, autotuned and
@@ -268,12 +269,13 @@ def fused_leg(QReg, n, ops, warmup, timed_gates, plain_value, dense=0):
     return out
 
 
-def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0):
+def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0, per_step_steps=0):
     """Transverse-field Ising Trotter steps on a ring (rzz on every edge, then
     rx on every qubit; J = 1, dt = 0.05, h_i ~ U[0.5, 1.5]); a step is 2n
     SingleOps.  The plain form runs plain_steps steps; the fused form runs
-    fused_steps steps as one call (and, with per_step, one call per step).
-    One register serves every form (reset to |0> in between)."""
+    fused_steps steps as one call (and one call per step: fused_steps steps
+    with per_step, else per_step_steps steps).  One register serves every
+    form (reset to |0> in between)."""
     from qvnt_amd import QReg, circuits, op
     step = circuits.build(circuits.tfim_step(n), op)
     amp = 16 if prec == "f64" else 8
@@ -306,8 +308,10 @@ def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0):
             log("tfim plain: %d SingleOps in %.2f s -> %.2f gates/s, k_%s %.0f GB/s; |psi|^2-1 = %.1e"
                 % (gates, wall, gates / wall, dom, ach, norm - 1.0))
         out["fused"] = {}
-        forms = ([("per_step", 1)] if per_step else []) + [("whole", fused_steps)]
-        for label, per_call in forms if fused_steps > 0 else ():
+        forms = [("whole", fused_steps, fused_steps)]
+        if per_step or per_step_steps:
+            forms.insert(0, ("per_step", 1, fused_steps if per_step else per_step_steps))
+        for label, per_call, steps in forms if fused_steps > 0 else ():
             full = step.clone()
             for _ in range(per_call - 1):
                 full *= step
@@ -318,18 +322,18 @@ def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0):
             reg.kernel_stats(reset=True)
             passes = 0
             t0 = time.perf_counter()
-            for _ in range(fused_steps // per_call):
+            for _ in range(steps // per_call):
                 passes += reg.apply_fused(full)
             reg.sync()
             wall = time.perf_counter() - t0
             st = reg.kernel_stats(reset=True)
             reg.set_timing(False)
             norm = reg.get_absolute()
-            gates = fused_steps * len(step)
+            gates = steps * len(step)
             f = st["fused"]
             ach = f["bytes"] / (f["ms"] * 1e-3) / 1e9 if f["ms"] else 0.0
             plain_v = out.get("plain", {}).get("value")
-            out["fused"][label] = {"value": round(gates / wall, 3), "unit": "gates/s", "steps": fused_steps,
+            out["fused"][label] = {"value": round(gates / wall, 3), "unit": "gates/s", "steps": steps,
                                    "passes": passes, "seconds": round(wall, 3),
                                    "speedup_vs_plain": round(gates / wall / plain_v, 2) if plain_v else None,
                                    "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
@@ -338,7 +342,7 @@ def tfim_leg(n, prec, plain_steps, fused_steps, per_step=False, dense=0):
                                                 "avg_ms": round(f["ms"] / f["launches"], 3) if f["launches"] else None},
                                    "norm_minus_1": norm - 1.0}
             log("tfim fused (%s): %d steps, %d passes, %.2f s -> %.1f gates/s, k_fused %.0f GB/s; |psi|^2-1 = %.1e"
-                % (label, fused_steps, passes, wall, gates / wall, ach, norm - 1.0))
+                % (label, steps, passes, wall, gates / wall, ach, norm - 1.0))
     finally:
         reg.close()
     return out
@@ -880,7 +884,8 @@ def main():
     extra = {}
     if not args.no_extra:
         # each leg frees its device memory before the next allocates
-        extra["tfim"] = {"f64": tfim_leg(34, "f64", 1, 50), "f32": tfim_leg(35, "f32", 1, 50)}
+        extra["tfim"] = {"f64": tfim_leg(34, "f64", 1, 50, per_step_steps=4),
+                         "f32": tfim_leg(35, "f32", 1, 50, per_step_steps=4)}
         extra["tfim_pair"] = tfim_pair_leg(28, 50)
         extra["batch"] = {"f64": batch_leg(4096, 20, "f64", 0, 0, 20, 2, 64),
                           "f32": batch_leg(4096, 20, "f32", 0, 0, 20, 2, 64, plain=False)}
