@@ -186,6 +186,31 @@ def test_auto_tile_28q(prec):
             assert infid_b <= 1e-5 and infid_c <= 1e-5
 
 
+def test_fp32_tiles_of_13_at_30_qubits():
+    """FP32 tiles of 13 qubits (64 KiB of LDS, 512 threads), which the tile
+    choice takes for one-layer calls: three brickwork layers at 30 qubits,
+    one call per layer (3 passes each: rows 0..3 and 9 free qubits a pass)
+    and one call for all three, against the plain form on the device."""
+    n = 30
+    layers = [circuits.build(circuits.brickwork_layer(n, l), op) for l in range(3)]
+    whole = layers[0].clone()
+    for o in layers[1:]:
+        whole *= o
+    say("fp32 tiles of 13 at 30 q")
+    with QReg(n, prec="f32") as a, QReg(n, prec="f32") as b, QReg(n, prec="f32") as c:
+        passes = 0
+        for o in layers:
+            a.apply(o)
+            passes += b.apply_fused(o, max_qubits=13)
+        pc = c.apply_fused(whole, max_qubits=13)
+        err_b, infid_b = a.compare(b)[:2]
+        err_c, infid_c = a.compare(c)[:2]
+    say("  passes %d per layer, %d whole; max |d| %.2e / %.2e, infidelity %.1e / %.1e"
+        % (passes, pc, err_b, err_c, infid_b, infid_c))
+    assert passes == 9
+    assert infid_b <= 1e-5 and infid_c <= 1e-5 and err_b <= 1e-5 and err_c <= 1e-5
+
+
 def test_fused_30q_round_trip():
     n = 30
     say("fused round trip n=30")
