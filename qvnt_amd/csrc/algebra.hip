@@ -274,10 +274,8 @@ int qvnt_reg_set_num(qvnt_reg* r, uint32_t q_num) {
     const uint64_t len = 1ull << ns;
     const uint64_t keep = std::min<uint64_t>(r->len, 1ull << q_num);
     void* psi = nullptr;
-    if (hipMalloc(&psi, len * r->amp_bytes()) != hipSuccess) {
-        (void)hipGetLastError();
+    if (dev_alloc(r->device, &psi, len * r->amp_bytes()) != QVNT_OK)
         return fail(QVNT_E_OOM, "device memory for the resized state");
-    }
     if (r->prec == QVNT_F64)
         hipLaunchKernelGGL(k_resize<double>, grid_for(len), dim3(kBlock), 0, r->stream, static_cast<c64*>(psi), len,
                            static_cast<const c64*>(r->psi), keep);
@@ -287,10 +285,11 @@ int qvnt_reg_set_num(qvnt_reg* r, uint32_t q_num) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
     if (e != hipSuccess) {
-        (void)hipFree(psi);
+        (void)hipStreamSynchronize(r->stream);
+        dev_release(r->device, psi, len * r->amp_bytes());
         return fail(QVNT_E_HIP, std::string("set_num: ") + hipGetErrorString(e));
     }
-    (void)hipFree(r->psi);
+    dev_release(r->device, r->psi, r->len * r->amp_bytes());
     r->psi = psi;
     r->q_num = q_num;
     r->ns = ns;
@@ -303,11 +302,9 @@ int qvnt_reg_get_polar(qvnt_reg* r, double* dst, uint64_t len) {
     const uint64_t n = 1ull << r->q_num;
     if (!dst || len < 2 * n) return fail(QVNT_E_ARG, "destination must hold 2 * 2^n doubles");
     const uint64_t chunk = std::min<uint64_t>(n, 1ull << 23);
-    double* buf = nullptr;
-    if (hipMalloc(&buf, 2 * chunk * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(QVNT_E_OOM, "polar staging buffer");
-    }
+    void* vbuf = nullptr;
+    if (dev_alloc(r->device, &vbuf, 2 * chunk * sizeof(double)) != QVNT_OK) return fail(QVNT_E_OOM, "polar staging buffer");
+    double* buf = static_cast<double*>(vbuf);
     int rc = QVNT_OK;
     for (uint64_t off = 0; off < n && rc == QVNT_OK; off += chunk) {
         const uint64_t m = std::min(chunk, n - off);
@@ -322,7 +319,8 @@ int qvnt_reg_get_polar(qvnt_reg* r, double* dst, uint64_t len) {
         if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
         if (e != hipSuccess) rc = fail(QVNT_E_HIP, std::string("get_polar: ") + hipGetErrorString(e));
     }
-    (void)hipFree(buf);
+    if (rc != QVNT_OK) (void)hipStreamSynchronize(r->stream);
+    dev_release(r->device, buf, 2 * chunk * sizeof(double));
     return rc;
 }
 

@@ -890,10 +890,20 @@ int run_program(qvnt_reg* r, const Program& prog) {
     const size_t total = program_bytes(prog);
     if (total > r->prog_cap) {
         QVNT_HIP(hipStreamSynchronize(r->stream));
-        if (r->prog) QVNT_HIP(hipFree(r->prog));
+        void* old = r->prog;
+        const size_t old_cap = r->prog_cap;
         r->prog = nullptr;
         r->prog_cap = 0;
-        if (hipMalloc(&r->prog, total) != hipSuccess) return fail(QVNT_E_OOM, "device memory for the fused program");
+        // (a register that owns its memory takes its blocks from the device's
+        // cache and gives them back there; the stream is synchronized above)
+        if (r->owns_memory) {
+            dev_release(r->device, old, old_cap);
+            if (dev_alloc(r->device, &r->prog, total) != QVNT_OK)
+                return fail(QVNT_E_OOM, "device memory for the fused program");
+        } else {
+            if (old) QVNT_HIP(hipFree(old));
+            if (hipMalloc(&r->prog, total) != hipSuccess) return fail(QVNT_E_OOM, "device memory for the fused program");
+        }
         r->prog_cap = total;
     }
     host.resize(total);

@@ -238,24 +238,63 @@ TreeInfo tree_info(uint64_t count) {
     return t;
 }
 
+// Workspace blocks of a register that owns its memory come from and go
+// back to the device's block cache (reg.hip); the register's stream is
+// synchronized before a block goes back, so nothing queued still uses it.
+int ws_alloc(qvnt_reg* r, void** p, size_t bytes) {
+    if (r->owns_memory) return dev_alloc(r->device, p, bytes);
+    if (hipMalloc(p, bytes) == hipSuccess) return QVNT_OK;
+    (void)hipGetLastError();
+    *p = nullptr;
+    return QVNT_E_OOM;
+}
+
+int ws_release(qvnt_reg* r, void* p, size_t bytes) {
+    if (!p) return QVNT_OK;
+    if (!r->owns_memory) {
+        QVNT_HIP(hipFree(p));
+        return QVNT_OK;
+    }
+    QVNT_HIP(hipStreamSynchronize(r->stream));
+    dev_release(r->device, p, bytes);
+    return QVNT_OK;
+}
+
 int ensure_workspace(qvnt_reg* r, size_t tree_doubles, size_t shots) {
     if (tree_doubles > r->tree_cap) {
-        if (r->tree) QVNT_HIP(hipFree(r->tree));
+        void* old = r->tree;
+        const size_t old_bytes = r->tree_cap * sizeof(double);
         r->tree = nullptr;
         r->tree_cap = 0;
-        if (hipMalloc(&r->tree, tree_doubles * sizeof(double)) != hipSuccess)
+        if (int rc = ws_release(r, old, old_bytes)) return rc;
+        void* p = nullptr;
+        if (ws_alloc(r, &p, tree_doubles * sizeof(double)) != QVNT_OK)
             return fail(QVNT_E_OOM, "device memory for the measurement tree");
+        r->tree = static_cast<double*>(p);
         r->tree_cap = tree_doubles;
     }
     if (shots > r->shots_cap || !r->dscratch) {
         const size_t cap = shots < 64 ? 64 : shots;
-        if (r->dscratch) QVNT_HIP(hipFree(r->dscratch));
-        if (r->iscratch) QVNT_HIP(hipFree(r->iscratch));
+        // (both blocks were sized by shots_cap; it is cleared before either
+        // goes, so a failure below leaves no block recorded at a wrong size)
+        void* d_old = r->dscratch;
+        void* i_old = r->iscratch;
+        const size_t old_cap = r->shots_cap;
         r->dscratch = nullptr;
         r->iscratch = nullptr;
-        if (hipMalloc(&r->dscratch, (cap + 1) * sizeof(double)) != hipSuccess ||
-            hipMalloc(&r->iscratch, cap * sizeof(uint64_t)) != hipSuccess)
+        r->shots_cap = 0;
+        if (int rc = ws_release(r, d_old, (old_cap + 1) * sizeof(double))) return rc;
+        if (int rc = ws_release(r, i_old, old_cap * sizeof(uint64_t))) return rc;
+        void* d = nullptr;
+        void* i = nullptr;
+        if (ws_alloc(r, &d, (cap + 1) * sizeof(double)) != QVNT_OK)
             return fail(QVNT_E_OOM, "device memory for measurement scratch");
+        if (ws_alloc(r, &i, cap * sizeof(uint64_t)) != QVNT_OK) {
+            (void)ws_release(r, d, (cap + 1) * sizeof(double));
+            return fail(QVNT_E_OOM, "device memory for measurement scratch");
+        }
+        r->dscratch = static_cast<double*>(d);
+        r->iscratch = static_cast<uint64_t*>(i);
         r->shots_cap = cap;
     }
     return QVNT_OK;
@@ -398,8 +437,10 @@ int rescale(qvnt_reg* r, double scale) {
 int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n) {
     // bounded device staging: chunks of up to 2^24 probabilities
     const uint64_t chunk = n < (1ull << 24) ? n : (1ull << 24);
-    double* buf = nullptr;
-    if (hipMalloc(&buf, chunk * sizeof(double)) != hipSuccess) return fail(QVNT_E_OOM, "probability staging buffer");
+    void* vbuf = nullptr;
+    if (dev_alloc(r->device, &vbuf, chunk * sizeof(double)) != QVNT_OK)
+        return fail(QVNT_E_OOM, "probability staging buffer");
+    double* buf = static_cast<double*>(vbuf);
     int rc = QVNT_OK;
     for (uint64_t off = 0; off < n && rc == QVNT_OK; off += chunk) {
         const uint64_t m = (n - off) < chunk ? (n - off) : chunk;
@@ -413,16 +454,20 @@ int probabilities(qvnt_reg* r, double inv_total, double* dst, uint64_t n) {
         if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
         if (e != hipSuccess) rc = fail(QVNT_E_HIP, std::string("probabilities: ") + hipGetErrorString(e));
     }
-    (void)hipFree(buf);
+    // (every chunk ended with a synchronization; after a failure, wait here)
+    if (rc != QVNT_OK) (void)hipStreamSynchronize(r->stream);
+    dev_release(r->device, buf, chunk * sizeof(double));
     return rc;
 }
 
 int compare(qvnt_reg* a, qvnt_reg* b, double* out) {
     const uint64_t len = a->len;
     const unsigned blocks = (unsigned)((len + 4095) / 4096);
-    double* part = nullptr;
     QVNT_HIP(hipStreamSynchronize(b->stream));
-    if (hipMalloc(&part, (size_t)blocks * 5 * sizeof(double)) != hipSuccess) return fail(QVNT_E_OOM, "comparison partials");
+    void* vpart = nullptr;
+    if (dev_alloc(a->device, &vpart, (size_t)blocks * 5 * sizeof(double)) != QVNT_OK)
+        return fail(QVNT_E_OOM, "comparison partials");
+    double* part = static_cast<double*>(vpart);
     if (a->prec == QVNT_F64 && b->prec == QVNT_F64) launch_compare<double, double>(a->psi, b->psi, len, part, blocks, a->stream);
     else if (a->prec == QVNT_F64) launch_compare<double, float>(a->psi, b->psi, len, part, blocks, a->stream);
     else if (b->prec == QVNT_F64) launch_compare<float, double>(a->psi, b->psi, len, part, blocks, a->stream);
@@ -431,7 +476,8 @@ int compare(qvnt_reg* a, qvnt_reg* b, double* out) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host.data(), part, host.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
-    (void)hipFree(part);
+    if (e != hipSuccess) (void)hipStreamSynchronize(a->stream);
+    dev_release(a->device, part, (size_t)blocks * 5 * sizeof(double));
     if (e != hipSuccess) return fail(QVNT_E_HIP, std::string("compare: ") + hipGetErrorString(e));
     double s[5] = {0, 0, 0, 0, 0};
     for (unsigned k = 0; k < blocks; ++k) {

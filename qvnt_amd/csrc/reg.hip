@@ -2,10 +2,14 @@
 // get_absolute / get_probabilities / measure_mask (src/register/quant.rs),
 // plus state copy-in/out and per-kernel-class timing.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "reg.hpp"
 
@@ -33,6 +37,117 @@ Timed::~Timed() {
 
 namespace {
 
+constexpr int kMaxDevices = 64;
+constexpr size_t kStreamPool = 64;
+
+struct DeviceCache {
+    std::mutex mu;
+    std::multimap<size_t, void*> blocks;  // by size
+    size_t total = 0;
+    std::vector<hipStream_t> streams;     // idle
+};
+
+DeviceCache& cache_of(int device) {
+    static DeviceCache caches[kMaxDevices];
+    return caches[device];
+}
+
+bool cached_device(int device) { return device >= 0 && device < kMaxDevices; }
+
+void flush_blocks(int device) {
+    if (!cached_device(device)) return;
+    DeviceCache& c = cache_of(device);
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto& kv : c.blocks) (void)hipFree(kv.second);
+    c.blocks.clear();
+    c.total = 0;
+}
+
+// Makes `device` current for its lifetime, so that a new block or stream is
+// the device's own whatever device the calling thread has current.
+struct OnDevice {
+    int prev = -1;
+    explicit OnDevice(int device) {
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != device && hipSetDevice(device) == hipSuccess) prev = cur;
+    }
+    ~OnDevice() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+int dev_alloc(int device, void** p, size_t bytes) {
+    *p = nullptr;
+    if (cached_device(device) && bytes <= kCacheBlock) {
+        DeviceCache& c = cache_of(device);
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.blocks.find(bytes);
+        if (it != c.blocks.end()) {
+            *p = it->second;
+            c.total -= bytes;
+            c.blocks.erase(it);
+            return QVNT_OK;
+        }
+    }
+    OnDevice on(device);
+    if (hipMalloc(p, bytes) == hipSuccess) return QVNT_OK;
+    (void)hipGetLastError();
+    flush_blocks(device);
+    if (hipMalloc(p, bytes) == hipSuccess) return QVNT_OK;
+    (void)hipGetLastError();
+    *p = nullptr;
+    return QVNT_E_OOM;
+}
+
+void dev_release(int device, void* p, size_t bytes) {
+    if (!p) return;
+    if (cached_device(device) && bytes <= kCacheBlock) {
+        DeviceCache& c = cache_of(device);
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.total + bytes <= kCacheTotal) {
+            c.blocks.emplace(bytes, p);
+            c.total += bytes;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+int stream_acquire(int device, hipStream_t* s) {
+    *s = nullptr;
+    if (cached_device(device)) {
+        DeviceCache& c = cache_of(device);
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (!c.streams.empty()) {
+            *s = c.streams.back();
+            c.streams.pop_back();
+            return QVNT_OK;
+        }
+    }
+    OnDevice on(device);
+    if (hipStreamCreateWithFlags(s, hipStreamNonBlocking) == hipSuccess) return QVNT_OK;
+    (void)hipGetLastError();
+    *s = nullptr;
+    return QVNT_E_HIP;
+}
+
+void stream_release(int device, hipStream_t s) {
+    if (!s) return;
+    if (cached_device(device)) {
+        DeviceCache& c = cache_of(device);
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.streams.size() < kStreamPool) {
+            c.streams.push_back(s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
+}
+
+namespace {
+
 int fold_timing(qvnt_reg* r) {
     if (r->pending.empty()) return QVNT_OK;
     QVNT_HIP(hipStreamSynchronize(r->stream));
@@ -51,15 +166,26 @@ int fold_timing(qvnt_reg* r) {
 }
 
 // Only gfx950 code objects are built; refuse any other device up front.
+// The device count and each device's architecture are read once per
+// process (the count query alone cost ~65 us a register).
 int check_device(int* dev) {
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
-        return fail(QVNT_E_NO_DEVICE, "no HIP device visible");
+    static const int count = [] {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) {
+            (void)hipGetLastError();
+            c = 0;
+        }
+        return c;
+    }();
+    if (count == 0) return fail(QVNT_E_NO_DEVICE, "no HIP device visible");
     QVNT_HIP(hipGetDevice(dev));
+    static std::atomic<bool> gfx950[kMaxDevices];
+    if (cached_device(*dev) && gfx950[*dev].load(std::memory_order_acquire)) return QVNT_OK;
     hipDeviceProp_t prop;
     QVNT_HIP(hipGetDeviceProperties(&prop, *dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(QVNT_E_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    if (cached_device(*dev)) gfx950[*dev].store(true, std::memory_order_release);
     return QVNT_OK;
 }
 
@@ -86,13 +212,12 @@ int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg
     r->len = 1ull << r->ns;
     r->prec = prec;
     r->device = dev;
-    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (stream_acquire(dev, &r->stream) != QVNT_OK) {
         delete r;
         return fail(QVNT_E_HIP, "stream creation failed");
     }
-    if (hipMalloc(&r->psi, r->len * r->amp_bytes()) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipStreamDestroy(r->stream);
+    if (dev_alloc(dev, &r->psi, r->len * r->amp_bytes()) != QVNT_OK) {
+        stream_release(dev, r->stream);
         delete r;
         return fail(QVNT_E_OOM, "device memory for the state vector");
     }
@@ -124,12 +249,13 @@ int qvnt_reg_free(qvnt_reg* r) {
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
     if (r->owns_memory) {
-        if (r->psi) (void)hipFree(r->psi);
-        if (r->tree) (void)hipFree(r->tree);
-        if (r->dscratch) (void)hipFree(r->dscratch);
-        if (r->iscratch) (void)hipFree(r->iscratch);
-        if (r->prog) (void)hipFree(r->prog);
-        if (r->stream) (void)hipStreamDestroy(r->stream);
+        // (the stream is synchronized: nothing queued uses these any more)
+        dev_release(r->device, r->psi, r->len * r->amp_bytes());
+        dev_release(r->device, r->tree, r->tree_cap * sizeof(double));
+        dev_release(r->device, r->dscratch, (r->shots_cap + 1) * sizeof(double));
+        dev_release(r->device, r->iscratch, r->shots_cap * sizeof(uint64_t));
+        dev_release(r->device, r->prog, r->prog_cap);
+        stream_release(r->device, r->stream);
     }
     delete r;
     return QVNT_OK;

@@ -60,6 +60,24 @@ struct Timed {
     ~Timed();
 };
 
+// reg.hip: device blocks and streams that registers reuse.  Creating and
+// destroying a stream and allocating and freeing the state and workspace
+// cost more than the gates of a short-lived register (a 20-qubit register
+// with a few gates and a measurement: most of ~290 us).  dev_alloc returns a
+// cached block of exactly `bytes` of the device (current) or a new one;
+// dev_release keeps a block of at most kCacheBlock bytes while the
+// device's cache holds at most kCacheTotal, and frees the rest.  The caller
+// of dev_release guarantees that no queued work still uses the block (its
+// stream synchronized).  An allocation that fails frees every cached block
+// of the device and tries once more.  stream_acquire / stream_release do
+// the same for idle streams (default priority, non-blocking).
+constexpr size_t kCacheBlock = size_t(64) << 20;
+constexpr size_t kCacheTotal = size_t(256) << 20;
+int dev_alloc(int device, void** p, size_t bytes);
+void dev_release(int device, void* p, size_t bytes);
+int stream_acquire(int device, hipStream_t* s);
+void stream_release(int device, hipStream_t s);
+
 // plain.hip
 // Validates a gate against a buffer of `len` amplitudes (bit counts, control
 // overlap, qubits beyond the buffer); *noop is set for gates that are the

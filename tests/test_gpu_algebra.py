@@ -171,3 +171,43 @@ def test_sym_teleport_like():
     assert abs(probs[0b011] - 0.5) < 1e-12 and abs(probs[0b111] - 0.5) < 1e-12
     s.reset()
     assert s.get_class().get() == 0 and s.get_probabilities()[0] == 1.0
+
+
+def test_reused_blocks_and_streams_start_clean(orc):
+    """A freed register hands its state, workspace and stream back to the
+    device's cache and the next register of the same sizes takes them: a
+    register made on a reused block holds exactly its basis state, a resize
+    onto a reused (stale) block holds the kept amplitudes and zeros, and a
+    circuit with sampling and a measurement repeats bit for bit on reused
+    blocks, also when the sampling workspace grows mid-life."""
+    from qvnt_amd import circuits
+    (n, specs), = circuits.batched_registers(1, seed=5, n_lo=12, n_hi=12, g_lo=60, g_hi=60)
+    ops = circuits.build(specs, op)
+    small = np.array([qvnt_amd.uniform(5, 0, j) for j in range(300)])
+    big = np.array([qvnt_amd.uniform(5, 0, j) for j in range(5000)])
+    first = None
+    for it in range(4):
+        s = 0x5A5 if it % 2 == 0 else 0x0F3
+        with QReg.with_state(n, s) as r:
+            psi = r.get_state()
+            assert psi[s] == 1 and np.count_nonzero(psi) == 1, "iteration %d" % it
+            if it % 2:
+                continue
+            r.apply(ops)
+            r.apply_fused(ops)
+            a = r.sample_mask(0xFF0, small)
+            b = r.sample_mask(0xFF0, big)
+            assert np.array_equal(b[:300], a)
+            m = r.measure_mask(0b1010, u=0.25).get()
+            got = (r.get_state(), a, b, m)
+        if first is None:
+            first = got
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(got, first)), "iteration %d" % it
+    with QReg.with_state(n, 0) as r:
+        r.apply(ops)            # leaves a scrambled 12-qubit block in the cache
+    o = orc.Reg(10, 3)
+    o.set_num(n)
+    with QReg.with_state(10, 3) as r:
+        r.set_num(n)
+        assert np.array_equal(r.get_state(), o.state())
