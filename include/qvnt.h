@@ -167,7 +167,18 @@ typedef struct qvnt_reg qvnt_reg;
 int qvnt_reg_new(uint32_t q_num, qvnt_prec prec, qvnt_reg** out);
 /* QReg::with_state (quant.rs:129-150): psi[state & q_mask] = 1. */
 int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg** out);
+/* Drop (quant.rs: the Vec is freed).  Waits for the register's queued work;
+ * its stream and its device blocks of at most 64 MiB then go to a
+ * per-device cache (at most 64 idle streams and 256 MiB of blocks), from
+ * which the next register of the same sizes takes them: creating a stream
+ * and allocating cost more than the gates of a short-lived register.  An
+ * allocation that fails empties the device's block cache and tries once
+ * more. */
 int qvnt_reg_free(qvnt_reg* reg);
+/* Frees every cached block and idle stream of every device.  No counterpart
+ * in the reference (it keeps no cache): for callers that share the GPU's
+ * memory with another allocator and want the cache's memory back. */
+int qvnt_cache_trim(void);
 uint32_t qvnt_reg_num(const qvnt_reg* reg);                       /* QReg::num */
 /* Number of stored amplitudes, max(2^n, 8). */
 uint64_t qvnt_reg_len(const qvnt_reg* reg);

@@ -6,10 +6,10 @@ The package loads ``qvnt_amd/lib/libqvnt_hip.so``; there is no CPU path.
 """
 from ._lib import QvntError, Gate, KIND, KINDS, lib, last_error
 from .ops import MultiOp, from_gates, op
-from .register import CReg, QReg, VReg, fused_plan_stats, uniform
+from .register import CReg, QReg, VReg, cache_trim, fused_plan_stats, uniform
 from .program import Program, Sym
 from . import qasm
 
 __all__ = ["QvntError", "Gate", "KIND", "KINDS", "MultiOp", "from_gates", "op",
-           "CReg", "QReg", "VReg", "Program", "Sym", "qasm", "fused_plan_stats", "uniform", "lib",
+           "CReg", "QReg", "VReg", "Program", "Sym", "qasm", "fused_plan_stats", "cache_trim", "uniform", "lib",
            "last_error"]

@@ -81,6 +81,7 @@ def _declare(L):
         "qvnt_reg_new": ([u32, i, opp], i),
         "qvnt_reg_with_state": ([u32, u64, i, opp], i),
         "qvnt_reg_free": ([vp], i),
+        "qvnt_cache_trim": ([], i),
         "qvnt_reg_num": ([vp], u32),
         "qvnt_reg_len": ([vp], u64),
         "qvnt_reg_prec": ([vp], i),

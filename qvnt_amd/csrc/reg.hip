@@ -198,6 +198,17 @@ using namespace qvnt;
 
 extern "C" {
 
+int qvnt_cache_trim(void) {
+    for (int d = 0; d < kMaxDevices; ++d) {
+        flush_blocks(d);
+        DeviceCache& c = cache_of(d);
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (hipStream_t s : c.streams) (void)hipStreamDestroy(s);
+        c.streams.clear();
+    }
+    return QVNT_OK;
+}
+
 int qvnt_reg_with_state(uint32_t q_num, uint64_t state, qvnt_prec prec, qvnt_reg** out) {
     if (!out) return fail(QVNT_E_ARG, "null output pointer");
     if (prec != QVNT_F64 && prec != QVNT_F32) return fail(QVNT_E_ARG, "unknown precision");

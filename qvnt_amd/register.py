@@ -26,6 +26,12 @@ def fused_plan_stats(op, q_num, prec="f64", max_qubits=0, dense_min=0):
     return {"passes": int(out[0]), "tile": int(out[1]), "cost": out[2], "dense_phases": int(out[3])}
 
 
+def cache_trim():
+    """Give the device blocks and idle streams that freed registers left in
+    the library's per-device cache back to the device (qvnt_cache_trim)."""
+    check(lib.qvnt_cache_trim())
+
+
 def uniform(seed, stream, index):
     """The seeded draw in [0, 1) (SplitMix64; include/qvnt.h qvnt_uniform)."""
     return lib.qvnt_uniform(seed, stream, index)

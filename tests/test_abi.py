@@ -79,3 +79,9 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert ctypes.sizeof(py) == c[cname], cname
         for f in py._fields_:
             assert getattr(py, f[0]).offset == c["%s.%s" % (cname, f[0])], (cname, f[0])
+
+
+def test_cache_trim_without_a_device():
+    """Emptying the register cache needs no device and leaves none behind."""
+    qvnt_amd.cache_trim()
+    qvnt_amd.cache_trim()

@@ -211,3 +211,10 @@ def test_reused_blocks_and_streams_start_clean(orc):
     with QReg.with_state(10, 3) as r:
         r.set_num(n)
         assert np.array_equal(r.get_state(), o.state())
+    qvnt_amd.cache_trim()           # everything cached goes back to the device
+    with QReg.with_state(n, 0x5A5) as r:
+        r.apply(ops)
+        r.apply_fused(ops)
+        assert np.array_equal(r.sample_mask(0xFF0, big), first[2])
+        assert r.measure_mask(0b1010, u=0.25).get() == first[3]
+        assert np.array_equal(r.get_state(), first[0])       # (the state after the measurement)
