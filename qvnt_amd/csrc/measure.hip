@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -111,17 +112,18 @@ struct TreeInfo {
 };
 
 // One draw per block: descend the stored levels, then the recomputed
-// 2^lb-leaf subtree of the block reached.
+// 2^lb-leaf subtree of the block reached.  Draw k is u[k], or u0 when u is
+// null (one draw, passed by value: no copy to the device before the launch).
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_sample(const cplx<R>* __restrict__ psi, const double* __restrict__ tree,
-                                                   TreeInfo info, const double* __restrict__ u, uint64_t* __restrict__ out,
-                                                   double* __restrict__ total) {
+                                                   TreeInfo info, const double* __restrict__ u, double u0,
+                                                   uint64_t* __restrict__ out, double* __restrict__ total) {
     __shared__ double lv[2 * kLeaf];
     __shared__ uint64_t sj;
     __shared__ double sx;
     if (threadIdx.x == 0) {
         const double T = tree[info.off[info.top]];
-        double x = u[blockIdx.x] * T;
+        double x = (u ? u[blockIdx.x] : u0) * T;
         uint64_t j = 0;
         for (uint32_t l = info.top; l > info.lb; --l) {
             const double L = tree[info.off[l - 1] + 2 * j], Rr = tree[info.off[l - 1] + 2 * j + 1];
@@ -284,12 +286,12 @@ int ensure_workspace(qvnt_reg* r, size_t tree_doubles, size_t shots) {
         r->iscratch = nullptr;
         r->shots_cap = 0;
         if (int rc = ws_release(r, d_old, (old_cap + 1) * sizeof(double))) return rc;
-        if (int rc = ws_release(r, i_old, old_cap * sizeof(uint64_t))) return rc;
+        if (int rc = ws_release(r, i_old, (old_cap + 1) * sizeof(uint64_t))) return rc;
         void* d = nullptr;
         void* i = nullptr;
         if (ws_alloc(r, &d, (cap + 1) * sizeof(double)) != QVNT_OK)
             return fail(QVNT_E_OOM, "device memory for measurement scratch");
-        if (ws_alloc(r, &i, cap * sizeof(uint64_t)) != QVNT_OK) {
+        if (ws_alloc(r, &i, (cap + 1) * sizeof(uint64_t)) != QVNT_OK) {
             (void)ws_release(r, d, (cap + 1) * sizeof(double));
             return fail(QVNT_E_OOM, "device memory for measurement scratch");
         }
@@ -331,10 +333,11 @@ int build_levels(qvnt_reg* r, double* tree, uint64_t count, const TreeInfo& info
     return QVNT_OK;
 }
 
-// Tree levels plus one draw per u_dev[k] into out_dev, total into
-// total_dev[0]; every pointer is device memory; no synchronisation.
-int sample_into(qvnt_reg* r, double* tree, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev,
-                double* total_dev, bool timed) {
+// Tree levels plus one draw per u_dev[k] (u_dev null: the single draw u0)
+// into out_dev, total into total_dev[0]; every pointer is device memory; no
+// synchronisation.
+int sample_into(qvnt_reg* r, double* tree, uint64_t count, const double* u_dev, double u0, size_t shots,
+                uint64_t* out_dev, double* total_dev, bool timed) {
     const TreeInfo info = tree_info(count);
     int rc = r->prec == QVNT_F64 ? build_levels<double>(r, tree, count, info, timed)
                                  : build_levels<float>(r, tree, count, info, timed);
@@ -342,10 +345,10 @@ int sample_into(qvnt_reg* r, double* tree, uint64_t count, const double* u_dev, 
     Timed t(timed ? r : nullptr, QVNT_K_OTHER, 0.0);
     if (r->prec == QVNT_F64)
         hipLaunchKernelGGL(k_sample<double>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                           static_cast<const c64*>(r->psi), (const double*)tree, info, u_dev, out_dev, total_dev);
+                           static_cast<const c64*>(r->psi), (const double*)tree, info, u_dev, u0, out_dev, total_dev);
     else
         hipLaunchKernelGGL(k_sample<float>, dim3((unsigned)shots), dim3(kBlock), 0, r->stream,
-                           static_cast<const c32*>(r->psi), (const double*)tree, info, u_dev, out_dev, total_dev);
+                           static_cast<const c32*>(r->psi), (const double*)tree, info, u_dev, u0, out_dev, total_dev);
     QVNT_HIP(hipGetLastError());
     return QVNT_OK;
 }
@@ -373,18 +376,32 @@ size_t tree_doubles(uint64_t count) {
 
 int sample_launch(qvnt_reg* r, uint64_t count, const double* u_dev, size_t shots, uint64_t* out_dev, double* total_dev) {
     if (tree_doubles(count) > r->tree_cap) return fail(QVNT_E_ARG, "measurement tree workspace too small");
-    return sample_into(r, r->tree, count, u_dev, shots, out_dev, total_dev, true);
+    return sample_into(r, r->tree, count, u_dev, 0.0, shots, out_dev, total_dev, true);
 }
 
+// The outcomes land in iscratch[0, shots) and the total right after them
+// (iscratch holds shots_cap + 1 words), so one copy brings both back; a
+// single draw goes to the kernel by value.
 int tree_sample(qvnt_reg* r, uint64_t count, const double* u, size_t shots, uint64_t* out, double* total_out) {
+    if (shots == 0) return fail(QVNT_E_ARG, "no draws");
     int rc = ensure_workspace(r, tree_doubles(count), shots);
     if (rc) return rc;
-    QVNT_HIP(hipMemcpyAsync(r->dscratch + 1, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
-    rc = sample_launch(r, count, r->dscratch + 1, shots, r->iscratch, r->dscratch);
+    if (tree_doubles(count) > r->tree_cap) return fail(QVNT_E_ARG, "measurement tree workspace too small");
+    const double* u_dev = nullptr;
+    if (shots > 1) {
+        QVNT_HIP(hipMemcpyAsync(r->dscratch + 1, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
+        u_dev = r->dscratch + 1;
+    }
+    double* total_dev = reinterpret_cast<double*>(r->iscratch + shots);
+    rc = sample_into(r, r->tree, count, u_dev, shots > 1 ? 0.0 : u[0], shots, r->iscratch, total_dev, true);
     if (rc) return rc;
-    QVNT_HIP(hipMemcpyAsync(out, r->iscratch, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
-    QVNT_HIP(hipMemcpyAsync(total_out, r->dscratch, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    uint64_t one[2];
+    std::vector<uint64_t> many(shots > 1 ? shots + 1 : 0);
+    uint64_t* host = shots > 1 ? many.data() : one;
+    QVNT_HIP(hipMemcpyAsync(host, r->iscratch, (shots + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
     QVNT_HIP(hipStreamSynchronize(r->stream));
+    std::memcpy(out, host, shots * sizeof(uint64_t));
+    std::memcpy(total_out, host + shots, sizeof(double));
     return QVNT_OK;
 }
 
@@ -564,7 +581,7 @@ extern "C" int qvnt_reg_sample_mask_noalloc(qvnt_reg* r, uint64_t mask, const do
     double* u_dev = reinterpret_cast<double*>(ws + off_u);
     uint64_t* out_dev = reinterpret_cast<uint64_t*>(ws + off_out);
     QVNT_HIP(hipMemcpyAsync(u_dev, u, shots * sizeof(double), hipMemcpyHostToDevice, r->stream));
-    if (int rc = sample_into(r, tree, count, u_dev, shots, out_dev, total_dev, false)) return rc;
+    if (int rc = sample_into(r, tree, count, u_dev, 0.0, shots, out_dev, total_dev, false)) return rc;
     double total = 0.0;
     QVNT_HIP(hipMemcpyAsync(out, out_dev, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
     QVNT_HIP(hipMemcpyAsync(&total, total_dev, sizeof(double), hipMemcpyDeviceToHost, r->stream));

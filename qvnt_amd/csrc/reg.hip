@@ -264,7 +264,7 @@ int qvnt_reg_free(qvnt_reg* r) {
         dev_release(r->device, r->psi, r->len * r->amp_bytes());
         dev_release(r->device, r->tree, r->tree_cap * sizeof(double));
         dev_release(r->device, r->dscratch, (r->shots_cap + 1) * sizeof(double));
-        dev_release(r->device, r->iscratch, r->shots_cap * sizeof(uint64_t));
+        dev_release(r->device, r->iscratch, (r->shots_cap + 1) * sizeof(uint64_t));
         dev_release(r->device, r->prog, r->prog_cap);
         stream_release(r->device, r->stream);
     }

@@ -29,7 +29,7 @@ struct qvnt_reg {
     double* tree = nullptr;
     size_t tree_cap = 0;     // doubles
     double* dscratch = nullptr;      // small device scratch (draws, totals)
-    uint64_t* iscratch = nullptr;    // small device scratch (outcomes)
+    uint64_t* iscratch = nullptr;    // small device scratch: shots_cap outcomes + the total
     size_t shots_cap = 0;
 
     // fused-form program (phases, micro-ops, coefficients) on the device
