@@ -134,20 +134,18 @@ def cpu_baseline(n, threads, ks=(1, 4)):
             "layer_as_one_apply_gates_per_s": round(layer / (a + layer * b), 4) if a + layer * b > 0 else None}
 
 
-def reference_bench_leg(threads):
+def reference_bench_leg(threads, cpu=True):
     """The reference's own benchmark (benches/performance.rs:4-20): per
     iteration QReg::with_state(q, 0), apply(qft(0b0111) * qft(0b1110)),
     measure_mask(0b100), for q = 18, 19, 20 -- on the GPU (plain form,
     register created and freed every iteration, as the reference allocates
-    its Vec) and on the oracle on 1 and `threads` host threads.  Iterations
-    per second; each side runs for about half a second."""
-    import oracle
+    its Vec) and, with `cpu`, on the oracle on 1 and `threads` host threads.
+    Iterations per second; each side runs for about half a second."""
     from qvnt_amd import QReg, op, uniform
     out = {}
     mask = 0b100
     for q in (18, 19, 20):
         gops = op.qft(0b0111) * op.qft(0b1110)
-        cops = oracle.op.qft(0b0111) * oracle.op.qft(0b1110)
         row = {}
         it, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < 0.5 or it < 3:
@@ -157,7 +155,10 @@ def reference_bench_leg(threads):
             assert got & ~mask == 0
             it += 1
         row["gpu_iter_per_s"] = round(it / (time.perf_counter() - t0), 1)
-        for th in sorted({threads, 1}):
+        if cpu:
+            import oracle
+            cops = oracle.op.qft(0b0111) * oracle.op.qft(0b1110)
+        for th in (sorted({threads, 1}) if cpu else []):
             it, t0 = 0, time.perf_counter()
             while time.perf_counter() - t0 < 0.5 or it < 3:
                 ref = oracle.Reg(q, 0, threads=th)
@@ -769,7 +770,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="brickwork: skip the served / TFIM / FP32-vs-FP64 / batched legs")
     ap.add_argument("--workload", default="brickwork",
-                    choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles", "plain-classes"])
+                    choices=["brickwork", "tfim", "tfim-pair", "serve", "batch", "tiles", "plain-classes",
+                             "reference"])
     ap.add_argument("--registers", type=int, default=4096, help="batch: number of registers")
     ap.add_argument("--tile-cap", type=int, default=0, help="batch: tile cap (0: autotune)")
     ap.add_argument("--dense", type=int, default=None,
@@ -792,7 +794,7 @@ def main():
         args.dense = 3 if args.workload == "serve" else 0
     if args.qubits is None:
         args.qubits = {"brickwork": 30, "tfim": 34 if args.prec == "f64" else 35, "tfim-pair": 28, "serve": 30,
-                       "batch": 20, "tiles": 30, "plain-classes": 30}[args.workload]
+                       "batch": 20, "tiles": 30, "plain-classes": 30, "reference": 20}[args.workload]
 
     import qvnt_amd  # noqa: F401  (fails loudly without the HIP library)
     from qvnt_amd import QReg, circuits, op
@@ -809,6 +811,13 @@ def main():
         return tiles(args)
     if args.workload == "plain-classes":
         return plain_classes(args)
+    if args.workload == "reference":
+        # the reference's Criterion loop alone (its GPU side without --no-cpu
+        # adds the oracle's), e.g. for a HIP API trace of the per-register calls
+        host = host_info()
+        refb = reference_bench_leg(cpu_threads(host), cpu=not args.no_cpu)
+        print(json.dumps({"config": {"workload": refb["workload"]}, "reference_bench": refb}), flush=True)
+        return
 
     n = args.qubits
     depth = max(40, args.warmup + args.steps)   # the depth-40 brickwork circuit
