@@ -140,13 +140,19 @@ def reference_bench_leg(threads, cpu=True):
     measure_mask(0b100), for q = 18, 19, 20 -- on the GPU (plain form,
     register created and freed every iteration, as the reference allocates
     its Vec) and, with `cpu`, on the oracle on 1 and `threads` host threads.
-    Iterations per second; each side runs for about half a second."""
+    Iterations per second; each side runs for about half a second, the GPU
+    side after 20 untimed iterations (code objects loaded, a stream and the
+    register's blocks cached)."""
     from qvnt_amd import QReg, op, uniform
     out = {}
     mask = 0b100
     for q in (18, 19, 20):
         gops = op.qft(0b0111) * op.qft(0b1110)
         row = {}
+        for it in range(20):
+            with QReg.with_state(q, 0) as r:
+                r.apply(gops)
+                r.measure_mask(mask, u=uniform(1, q, it)).get()
         it, t0 = 0, time.perf_counter()
         while time.perf_counter() - t0 < 0.5 or it < 3:
             with QReg.with_state(q, 0) as r:
