@@ -37,6 +37,10 @@ inline void check(int rc) {
     if (rc != QVNT_OK) throw Error(rc, qvnt_last_error());
 }
 
+// Gives the device blocks and idle streams that freed registers left in the
+// library's per-device cache back to the device (qvnt_cache_trim).
+inline void cache_trim() { check(qvnt_cache_trim()); }
+
 class MultiOp {
 public:
     explicit MultiOp(qvnt_op* h) : h_(h) {}

@@ -206,6 +206,8 @@ int main(int argc, char** argv) {
         histogram();
         virtual_regs_sizes();
         bench_workload();
+        qvnt::cache_trim();  // freed registers' blocks and streams go back
+        bell();              // and new registers allocate afresh
     }
     std::printf("%s: %d failure(s)\n", mode.c_str(), failures);
     return failures ? 1 : 0;
