@@ -47,8 +47,11 @@ struct DeviceCache {
     std::vector<hipStream_t> streams;     // idle
 };
 
+// Never destroyed: a register freed while the process exits (after this
+// library's static destructors, from a host's own static objects) still
+// finds its device's cache.
 DeviceCache& cache_of(int device) {
-    static DeviceCache caches[kMaxDevices];
+    static DeviceCache* const caches = new DeviceCache[kMaxDevices];
     return caches[device];
 }
 
